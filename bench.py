@@ -1,0 +1,180 @@
+"""Throughput benchmark of the DMC propagation path on one MI355X.
+
+Workload (the metric's headline case): plain DMC on the triatomic Morse+bend PES
+(atoms [H, H, O], masses 1837.15 / 29156.95, De=0.1916, a=1.2, re=1.81, kb=0.16,
+theta0=104.5 deg), 1M walkers started at the equilibrium geometry + N(0, 0.05 bohr),
+dt = 5 a.u., discrete birth/death, FP32 coordinates and per-walker PES with FP64 weights
+and reductions, Philox draws.  A step is one fused kernel launch (move, PES, weights,
+branching, compaction, vref); value = walker-steps/s = sum over the K timed steps of the
+population entering each step, divided by the wall time between two device syncs.
+
+Also reported: the same case with importance sampling (Gaussian-product psi_T, drift and
+local energy from the trial kernels, Metropolis), the dominant kernel's roofline fraction
+(algorithmic bytes / average launch duration from HIP events, against 8 TB/s) and the CPU
+baseline (the oracle with the reference's process-pool PES manager on the host cores, on a
+bounded sample of the same workload).
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+import numpy as np  # noqa: E402
+
+HBM_PEAK_GBS = 8000.0
+METRIC = "walker-steps/s at 1M walkers, plain vs IS+surrogate; ZPE err cm^-1 vs FP64"
+# algorithmic HBM bytes per walker-step of the fused discrete step, FP32 triatomic:
+# read 9 coordinates (36 B) + write 9 coordinates per surviving copy (36 B on average,
+# since the mean copy count is 1); draws are generated in registers, energies and weights
+# never leave registers.
+BYTES_PER_WALKER_STEP = {('plain', 'fp32', 9): 72, ('plain', 'fp64', 9): 144, ('is', 'fp32', 9): 72}
+
+
+def log(*a):
+    print(*a, file=sys.stderr, flush=True)
+
+
+def make_sim(n, steps, precision='fp32', imp=False, seed=1234):
+    import pyvibdmc_amd as pv
+    from pyvibdmc_amd.impsamp import water_trial
+    t = np.deg2rad(104.5)
+    eq = np.array([[1.81, 0.0, 0.0], [1.81 * np.cos(t), 1.81 * np.sin(t), 0.0], [0.0, 0.0, 0.0]])
+    start = eq[None] + np.random.RandomState(seed).normal(0.0, 0.05, size=(n, 3, 3))
+    sim = pv.DMC_Sim(num_walkers=n, num_timesteps=steps, atoms=['H', 'H', 'O'], masses=[1837.15, 1837.15, 29156.95],
+                     delta_t=5, potential=pv.MorseBendTriatomic(), start_structures=start,
+                     imp_samp=water_trial() if imp else None, precision=precision, rng='philox', seed=seed,
+                     write_files=False, equil_steps=10 ** 9, wfn_every=10 ** 9, chkpt_every=10 ** 9)
+    return sim
+
+
+def run_timed(sim, warmup, steps, events_steps=0):
+    """Warm up, then time exactly `steps` fused steps between two device syncs.  Optionally
+    time `events_steps` more launches individually with HIP events on the launch stream."""
+    import torch
+    from pyvibdmc_amd import _native as N
+    lib = N.load()
+    st = N.stream_handle()
+    sim._start()
+    N.check(lib.pvd_sim_run(sim._h, warmup, st))
+    sim._raise_if_failed()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    N.check(lib.pvd_sim_run(sim._h, steps, st))
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t0
+    sim._raise_if_failed()
+    sim._fetch_traj(0, warmup + steps)
+    pops = sim._pop_vs_tau
+    # population entering step t = population after step t-1
+    entering = pops[warmup - 1:warmup + steps - 1]
+    ws = float(np.sum(entering))
+    kern_ms = None
+    if events_steps:
+        evs = [(torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)) for _ in range(events_steps)]
+        for a, b in evs:
+            a.record()
+            N.check(lib.pvd_sim_run(sim._h, 1, st))
+            b.record()
+        torch.cuda.synchronize()
+        sim._raise_if_failed()
+        kern_ms = float(np.mean([a.elapsed_time(b) for a, b in evs]))
+    return ws, wall, float(np.mean(entering)), kern_ms
+
+
+def cpu_baseline(n):
+    from oracle.cpu_ref import cpu_model, time_plain_triatomic
+    steps = 12 if n >= 500_000 else 40
+    rate, sec, ws, cores = time_plain_triatomic(num_walkers=n, steps=steps, warmup=1)
+    return {"value": rate, "unit": "walker-steps/s", "cores": cores, "kind": "port",
+            "sample": f"plain discrete triatomic DMC, FP64 NumPy oracle with a {cores}-process pool PES "
+                      f"(reference structure), {n} walkers x {steps} timed steps ({sec:.1f} s), "
+                      f"CPU: {cpu_model()}"}
+
+
+def load_traffic(key):
+    path = os.path.join(ROOT, 'profiles', 'traffic.json')
+    try:
+        with open(path) as fh:
+            return json.load(fh).get(key)
+    except (OSError, ValueError):
+        return None
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument('--gpus', type=int, default=1)
+    ap.add_argument('--steps', type=int, default=2000)
+    ap.add_argument('--warmup', type=int, default=200)
+    ap.add_argument('--walkers', type=int, default=1_000_000)
+    ap.add_argument('--precision', default='fp32', choices=['fp32', 'fp64'])
+    ap.add_argument('--no-cpu', action='store_true')
+    ap.add_argument('--no-variants', action='store_true')
+    ap.add_argument('--event-steps', type=int, default=200)
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py measures one GPU: the DMC walkers of this workload live on a single MI355X "
+              f"(--gpus {args.gpus} is not supported; multi-GPU sharding is described in DESIGN.md).",
+              file=sys.stderr)
+        sys.exit(2)
+    import torch
+    assert torch.cuda.is_available(), 'bench.py needs the GPU'
+    n = args.walkers
+    K, W = args.steps, args.warmup
+    total = W + K + args.event_steps + 2
+
+    log(f'[bench] plain {args.precision} {n} walkers: warmup {W}, timed {K}')
+    sim = make_sim(n, total, precision=args.precision)
+    ws, wall, mean_n, kern_ms = run_timed(sim, W, K, args.event_steps)
+    value = ws / wall
+    ms_per_step = wall * 1e3 / K
+    bpw = BYTES_PER_WALKER_STEP[('plain', args.precision, 9)]
+    roof = None
+    if kern_ms:
+        bytes_per_launch = bpw * mean_n
+        achieved = bytes_per_launch / (kern_ms * 1e-3) / 1e9
+        roof = {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                "frac": achieved / HBM_PEAK_GBS,
+                "traffic": load_traffic(f'k_discrete_{args.precision}_np9_{n}'),
+                "kernel": "k_discrete<float,9> (fused move+PES+branch+compaction+vref)" if args.precision == 'fp32'
+                else "k_discrete<double,9>",
+                "avg_launch_us": kern_ms * 1e3, "bytes_per_launch": bytes_per_launch,
+                "bytes_per_walker_step": bpw}
+    log(f'[bench] plain: {value:.4g} walker-steps/s, {ms_per_step * 1e3:.2f} us/step, kernel {kern_ms} ms')
+    del sim
+    torch.cuda.empty_cache()
+
+    variants = {}
+    if not args.no_variants:
+        log('[bench] importance-sampled variant')
+        sim = make_sim(n, total, precision=args.precision, imp=True)
+        ws2, wall2, mean2, kern2 = run_timed(sim, W, K, 0)
+        variants['is'] = {"value": ws2 / wall2, "unit": "walker-steps/s", "ms_per_step": wall2 * 1e3 / K,
+                          "config": "same PES and walkers, Gaussian-product psi_T (2 stretches + bend), drift and "
+                                    "local energy from the trial kernels, Metropolis accept, dt_eff"}
+        log(f'[bench] IS: {ws2 / wall2:.4g} walker-steps/s')
+        del sim
+        torch.cuda.empty_cache()
+
+    cpu = None
+    if not args.no_cpu:
+        log('[bench] CPU baseline sample')
+        cpu = cpu_baseline(n)
+        log(f'[bench] CPU: {cpu["value"]:.4g} walker-steps/s on {cpu["cores"]} cores')
+
+    out = {"metric": METRIC, "value": value, "unit": "walker-steps/s", "n_gpus": 1, "steps": K, "warmup": W,
+           "ms_per_step": ms_per_step, "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+           "dtype": "f32" if args.precision == 'fp32' else "f64", "data": "synthetic",
+           "config": {"workload": f"plain DMC, triatomic Morse+bend PES, {n} walkers, dt=5, discrete "
+                                  f"birth/death, {args.precision} coordinates with FP64 weights/reductions, "
+                                  f"Philox draws",
+                      "walkers": n, "mean_population": mean_n, "parallelism": "single GPU"},
+           "roofline": roof, "cpu_baseline": cpu, "variants": variants}
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == '__main__':
+    main()

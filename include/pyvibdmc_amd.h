@@ -1,0 +1,169 @@
+/*
+ * pyvibdmc_amd — C ABI of the MI355X (gfx950) DMC propagation library.
+ *
+ * The reference (rjdirisio/pyvibdmc) is pure Python; its hot path is the per-step loop of
+ * DMC_Sim (pyvibdmc/pyvibdmc.py:701-876) and the plugin calls it makes:
+ *   potential.getpot(cds)          (simulation_utilities/potential_manager.py:71-99)
+ *   imp_samp.drift(cds)            (simulation_utilities/imp_samp.py:21-26)
+ * The entry points below are what a binding of that path needs: a simulation handle that
+ * owns the walkers in HBM and advances them, a split path for energies computed outside
+ * the library, and stand-alone potential / trial-wave-function kernels.
+ *
+ * Conventions
+ *  - Every function returns an int status: 0 ok, 1 population outside [0.5,1.5]*N0,
+ *    3 non-finite or huge branching weight, 4 capacity, 5 unsupported continuous split,
+ *    6 device wait expired, 7 bad argument, 8 HIP runtime error.  pvd_last_error() gives
+ *    a thread-local message for the last failure.
+ *  - Codes 1 and 3 are the reference's ValueError("Massive walker birth or death
+ *    event!!!!!!! Dying...") (pyvibdmc.py:397-400, 409-413, 714-717).
+ *  - Step functions are asynchronous on `stream` (a hipStream_t, NULL = default stream);
+ *    device-side failures are reported by pvd_sim_status().
+ *  - Pointers documented "device" must be device memory of the current HIP device;
+ *    "host" pointers are ordinary host memory.  Inputs are borrowed for the call only.
+ *  - One simulation handle must not be used from two threads at once.
+ */
+#ifndef PYVIBDMC_AMD_H
+#define PYVIBDMC_AMD_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define PVD_MAX_COMP 96 /* components per walker (atoms * dims) */
+#define PVD_FUSED_MAX_COMP 18
+
+enum pvd_precision { PVD_FP64 = 0, PVD_FP32 = 1 };
+enum pvd_weighting { PVD_DISCRETE = 0, PVD_CONTINUOUS = 1 };
+enum pvd_pes_kind {
+  PVD_PES_EXTERNAL = 0,   /* energies supplied by the caller (split path)               */
+  PVD_PES_HARMONIC_X = 1, /* V = p0 * x^2, x = first component of atom 0                */
+  PVD_PES_MORSE_1D = 2,   /* V = p0 * (1 - exp(p1 * x))^2  (p1 = -alpha)                */
+  PVD_PES_MORSE_BEND = 3, /* idx = (h1, h2, o); p = (De, a, re, kb, theta0)             */
+  PVD_PES_CLUSTER6 = 4    /* [O,H,H,O,H,H]; p = (De,a,re,kb,theta0, De_oo,a_oo,re_oo, qO,qH) */
+};
+enum pvd_trial_kind { PVD_TRIAL_NONE = 0, PVD_TRIAL_HARMONIC_1D = 1, PVD_TRIAL_GAUSS = 2 };
+
+/* Device potential.  Replaces the user function behind Potential*.getpot
+ * (potential_manager.py:71-99, 144-174, 237-253). */
+typedef struct pvd_pes_desc {
+  int32_t kind;
+  int32_t idx[4];
+  double p[12];
+} pvd_pes_desc;
+
+/* Trial wave function psi_T = prod of Gaussians in bonds / angles, or the 1-D oscillator of
+ * sample_potentials/PythonPots/harm_trial_wfn.py:6-40.  Replaces call_trial / call_derivs
+ * (imp_samp_manager.py:92-139, 197-224). */
+typedef struct pvd_trial_desc {
+  int32_t kind, nb, na;
+  int32_t bi[8], bj[8];
+  int32_t ai[4], aj[4], ak[4];
+  double balpha[8], br0[8];
+  double aalpha[4], ath0[4];
+  double c0, c1, c2; /* HARMONIC_1D: (alpha/pi)^0.25, alpha, alpha^2 | GAUSS: c0 = ln norm */
+} pvd_trial_desc;
+
+/* Simulation parameters: the DMC_Sim keyword arguments on the propagation path
+ * (pyvibdmc.py:67-98, 132-297). */
+typedef struct pvd_config {
+  int32_t num_walkers;    /* N0                                                        */
+  int32_t capacity;       /* walker slots; 0 = 1.5*N0 + 1 rounded up to 256            */
+  int32_t n_atoms;        /* M                                                         */
+  int32_t n_dim;          /* d                                                         */
+  int32_t precision;      /* pvd_precision                                             */
+  int32_t weighting;      /* pvd_weighting                                             */
+  int32_t branch_every;   /* pyvibdmc.py:139                                           */
+  int32_t num_timesteps;  /* length of the per-step record                             */
+  double delta_t;         /* pyvibdmc.py:78                                            */
+  double alpha;           /* 1/(2 dt) or DEBUG_alpha, pyvibdmc.py:200-203              */
+  double thresh_lower;    /* continuous lower weight threshold, pyvibdmc.py:214-233    */
+  double sigma[PVD_MAX_COMP];    /* sqrt(dt/m) per component, pyvibdmc.py:199          */
+  double inv_mass[PVD_MAX_COMP]; /* 1/m per component, pyvibdmc.py:266, 274            */
+  uint64_t seed;          /* Philox key for the production draws                       */
+  pvd_pes_desc pes;
+  pvd_trial_desc trial;
+} pvd_config;
+
+typedef struct pvd_sim pvd_sim;
+
+const char* pvd_version(void);
+const char* pvd_last_error(void);
+
+/* --- simulation handle: DMC_Sim.__init__ / _initialize (pyvibdmc.py:67-297) ---------- */
+int pvd_sim_create(const pvd_config* cfg, pvd_sim** out);
+int pvd_sim_destroy(pvd_sim* sim);
+
+/* Load walkers (host, (n, M, d) float64 C order) and continuous weights (host or NULL).
+ * Starts the record at step `step0` (cur_timestep, pyvibdmc.py:85). */
+int pvd_sim_set_walkers(pvd_sim* sim, const double* coords, int32_t n, const double* weights,
+                        int64_t step0, void* stream);
+/* Copy the walkers out (host, capacity >= n*M*d); weights NULL for discrete. Synchronous. */
+int pvd_sim_get_walkers(pvd_sim* sim, double* coords, double* weights, int64_t max_walkers,
+                        int32_t* n, void* stream);
+
+/* First-step reference energy: V(x0) [+ local kinetic energy] then calc_vref
+ * (pyvibdmc.py:760-769).  The _external form takes V(x0) from device memory. */
+int pvd_sim_start(pvd_sim* sim, void* stream);
+int pvd_sim_start_external(pvd_sim* sim, const double* v_dev, void* stream);
+
+/* n_steps fused steps with Philox draws: move, PES, weights / branching, vref
+ * (pyvibdmc.py:772-848). */
+int pvd_sim_run(pvd_sim* sim, int32_t n_steps, void* stream);
+
+/* One step with the caller's draws (device, float64): displacements (n, M, d) as
+ * np.random.normal(0, sigma, (n, d, M)).transpose(0, 2, 1) (pyvibdmc.py:544-546),
+ * Metropolis uniforms (n) or NULL, branching uniforms (n) or NULL (pyvibdmc.py:392, 601). */
+int pvd_sim_step_injected(pvd_sim* sim, const double* disp, const double* u_metro,
+                          const double* u_branch, void* stream);
+
+/* Split path for energies computed outside the library (a host Python PES or a device
+ * surrogate): move the walkers in place, let the caller evaluate V on the moved walkers,
+ * then branch with those energies.  disp / u_metro / u_branch: device or NULL (Philox). */
+int pvd_sim_move(pvd_sim* sim, const double* disp, const double* u_metro, void* stream);
+int pvd_sim_branch_external(pvd_sim* sim, const double* v_dev, const double* u_branch,
+                            void* stream);
+/* Current walkers as SoA planes: component p of walker i at planes[p*stride + i];
+ * element type float64 (PVD_FP64) or float32 (PVD_FP32). */
+int pvd_sim_planes(pvd_sim* sim, void** planes, int64_t* stride, int32_t* n, void* stream);
+
+/* Synchronises `stream`; returns the device status as the return value and fills the
+ * step the failure happened at, the population and the current vref. */
+int pvd_sim_status(pvd_sim* sim, int32_t* status, int64_t* fail_step, int32_t* n, double* vref,
+                   int64_t* next_step, void* stream);
+/* Per-step record (host arrays of `count`): vref_vs_tau, pop_vs_tau, births (discrete) or
+ * number branched (continuous), deaths, cumulative effective time (IS) (pyvibdmc.py:674-682,
+ * 372-378). Any pointer may be NULL. */
+int pvd_sim_trajectory(pvd_sim* sim, int64_t t0, int32_t count, double* vref, double* pop,
+                       int32_t* births, int32_t* deaths, double* eff_t);
+
+/* Descendant weighting window (pyvibdmc.py:739-747, 663-672, 856-872).  end() writes
+ * desc_wts (host, n_parent), parent coords (host, (n_parent, M, d)) and parent weights
+ * (host or NULL). */
+int pvd_sim_desc_begin(pvd_sim* sim, void* stream);
+int pvd_sim_desc_end(pvd_sim* sim, double* desc_wts, double* parent, double* parent_wts,
+                     int32_t* n_parent, void* stream);
+
+/* --- stand-alone kernels on AoS walkers (n, M, d), element type by `precision` ------- */
+/* V(x): the device form of getpot. */
+int pvd_pes_eval(const pvd_pes_desc* pes, int32_t precision, int32_t n_atoms, int32_t n_dim,
+                 const void* coords, int64_t n, void* v_out, void* stream);
+/* grad_out[i] * dV/dx at each walker (backward of V; forces are -dV/dx). */
+int pvd_pes_vjp(const pvd_pes_desc* pes, int32_t precision, int32_t n_atoms, int32_t n_dim,
+                const void* coords, int64_t n, const void* grad_out, void* grad_coords,
+                void* stream);
+/* psi, ln psi, dpsi/psi and d2psi/psi (diagonal) at each walker (ImpSamp.drift). NULL
+ * outputs are skipped. */
+int pvd_trial_eval(const pvd_trial_desc* trial, int32_t precision, int32_t n_atoms,
+                   int32_t n_dim, const void* coords, int64_t n, void* psi, void* lnpsi,
+                   void* dlnpsi, void* d2psi_over_psi, void* stream);
+/* grad_out[i] * d ln psi / dx (backward of ln psi; recomputes the forward). */
+int pvd_trial_vjp(const pvd_trial_desc* trial, int32_t precision, int32_t n_atoms,
+                  int32_t n_dim, const void* coords, int64_t n, const void* grad_out,
+                  void* grad_coords, void* stream);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* PYVIBDMC_AMD_H */

@@ -1,0 +1,867 @@
+// C ABI of the DMC propagation library: simulation handle, launch dispatch and the
+// auxiliary kernels (layout transposes, first-step vref, descendant weights, stand-alone
+// potential / trial evaluation).  Declarations and conventions: include/pyvibdmc_amd.h.
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <string>
+
+#include "../../include/pyvibdmc_amd.h"
+#include "pvd_kernels.hpp"
+
+using namespace pvd;
+
+static_assert(sizeof(pvd_pes_desc) == sizeof(PesParams), "pes layout");
+static_assert(sizeof(pvd_trial_desc) == sizeof(TrialParams), "trial layout");
+static_assert(PVD_MAX_COMP == MAXNP, "component bound");
+
+namespace {
+
+thread_local std::string g_err;
+
+enum : int { E_BAD = 7, E_HIP = 8 };
+
+int fail(int code, const std::string& msg) {
+  g_err = msg;
+  return code;
+}
+
+#define HIPCHK(x)                                                                         \
+  do {                                                                                    \
+    hipError_t e_ = (x);                                                                  \
+    if (e_ != hipSuccess) return fail(E_HIP, std::string(#x) + ": " + hipGetErrorString(e_)); \
+  } while (0)
+
+const char* status_text(int st) {
+  switch (st) {
+    case ST_POP:
+    case ST_WEIGHT:
+      return "Massive walker birth or death event!!!!!!! Dying...";
+    case ST_CAPACITY:
+      return "walker capacity exceeded";
+    case ST_BRANCH:
+      return "continuous split outside the supported range (too many walkers below the "
+             "threshold in one step, or weights below the threshold chosen for splitting)";
+    case ST_SPIN:
+      return "a bounded device wait expired";
+    default:
+      return "ok";
+  }
+}
+
+// ------------------------------------------------------------------ auxiliary kernels
+template <typename R>
+__global__ void k_aos_to_planes(const double* aos, R* planes, long long n, int np, long long cap) {
+  const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  for (int p = 0; p < np; ++p) planes[p * cap + i] = (R)aos[i * np + p];
+}
+
+template <typename R>
+__global__ void k_planes_to_aos(const R* planes, double* aos, long long n, int np, long long cap) {
+  const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  for (int p = 0; p < np; ++p) aos[i * np + p] = (double)planes[p * cap + i];
+}
+
+template <typename R>
+__global__ void k_copy_planes(const R* src, R* dst, long long n, int np, long long cap) {
+  const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  for (int p = 0; p < np; ++p) dst[p * cap + i] = src[p * cap + i];
+}
+
+__global__ void k_iota(int* who, long long n) {
+  const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) who[i] = (int)i;
+}
+
+__global__ void k_desc_count(const int* who, long long n, int* cnt) {
+  const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) atomicAdd(&cnt[who[i]], 1);
+}
+
+// continuous descendant weights: single-member groups copy their weight exactly; larger
+// groups are summed in 2^-44 fixed point with integer atomics (order independent).
+constexpr double FIX_SCALE = 17592186044416.0;  // 2^44
+__global__ void k_desc_cont(const int* who, const double* w, long long n, const int* cnt,
+                            double* desc, unsigned long long* acc) {
+  const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const int q = who[i];
+  if (cnt[q] == 1) {
+    desc[q] = w[i];
+  } else {
+    const long long f = llrint(w[i] * FIX_SCALE);
+    atomicAdd(&acc[q], (unsigned long long)f);
+  }
+}
+
+__global__ void k_desc_finish(const int* cnt, const unsigned long long* acc, double* desc,
+                              long long np, int continuous) {
+  const long long q = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (q >= np) return;
+  const int c = cnt[q];
+  if (!continuous) {
+    desc[q] = (double)c;
+  } else if (c == 0) {
+    desc[q] = 0.0;
+  } else if (c > 1) {
+    desc[q] = (double)(long long)acc[q] / FIX_SCALE;
+  }
+}
+
+// first-step energies and vref (pyvibdmc.py:760-769, 651-661)
+template <typename R, int NP, bool IS, bool EXT>
+__global__ __launch_bounds__(TILE) void k_start(StepArgs<R> A, int continuous) {
+  __shared__ double s_d[TILE / PVD_WAVE];
+  __shared__ unsigned s_last;
+  const int par = A.par;
+  Ctrl* C = A.ctrl;
+  const int n = C->n[par];
+  const int ntiles = (n + TILE - 1) / TILE;
+  const int tile = blockIdx.x;
+  if (tile >= ntiles) return;
+  const long long i = (long long)tile * TILE + threadIdx.x;
+  double E = 0.0, w = 0.0;
+  if (i < n) {
+    if constexpr (EXT) {
+      E = A.V_ext[i];
+    }
+    if constexpr (!EXT || IS) {
+      R x[NP];
+      load_walker<R, NP>(A, i, x);
+      if constexpr (!EXT) E = (double)pes_eval<R, NP>(A.pes, x);
+      if constexpr (IS) {
+        TrialOut<R, NP> t;
+        trial_eval<R, NP>(A.trial, x, t);
+        const R ke = local_kin<R, NP>(A, t.d2);
+        if constexpr (EXT) {
+          E = E + (double)ke;
+        } else {
+          E = (double)((R)E + ke);
+        }
+      }
+    }
+    w = continuous ? A.w[i] : 1.0;
+  }
+  const double swe = block_sum<TILE>(w * E, s_d);
+  const double sw = block_sum<TILE>(w, s_d);
+  if (threadIdx.x == 0) {
+    A.rec[tile].s0 = swe;
+    A.rec[tile].s1 = sw;
+    fence_release();
+    s_last = fetch_add_u(&C->done[par], 1u);
+  }
+  __syncthreads();
+  if (s_last != (unsigned)(ntiles - 1)) return;
+  fence_acquire();
+  double a0 = 0.0, a1 = 0.0;
+  for (int t = threadIdx.x; t < ntiles; t += TILE) {
+    a0 += A.rec[t].s0;
+    a1 += A.rec[t].s1;
+  }
+  a0 = block_sum<TILE>(a0, s_d);
+  a1 = block_sum<TILE>(a1, s_d);
+  if (threadIdx.x == 0) {
+    C->done[par] = 0u;
+    if (!continuous) {
+      C->vref[par] = a0 / (double)n - (A.alpha * ((double)(n - (int)A.n0) / A.n0));
+    } else {
+      C->vref[par] = a0 / a1 - (A.alpha * ((a1 - A.n0) / A.n0));
+    }
+  }
+}
+
+// stand-alone kernels on AoS walkers
+template <typename R, int NP>
+__global__ void k_pes_eval(PesParams P, const R* x, long long n, R* v) {
+  const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  R xx[NP];
+#pragma unroll
+  for (int p = 0; p < NP; ++p) xx[p] = x[i * NP + p];
+  v[i] = pes_eval<R, NP>(P, xx);
+}
+
+template <typename R, int NP>
+__global__ void k_pes_vjp(PesParams P, const R* x, long long n, const R* go, R* gx) {
+  const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  R xx[NP], g[NP];
+#pragma unroll
+  for (int p = 0; p < NP; ++p) xx[p] = x[i * NP + p];
+  pes_grad<R, NP>(P, xx, g);
+  const R s = go[i];
+#pragma unroll
+  for (int p = 0; p < NP; ++p) gx[i * NP + p] = s * g[p];
+}
+
+template <typename R, int NP>
+__global__ void k_trial_eval(TrialParams T, const R* x, long long n, R* psi, R* lnpsi, R* d1, R* d2) {
+  const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  R xx[NP];
+#pragma unroll
+  for (int p = 0; p < NP; ++p) xx[p] = x[i * NP + p];
+  TrialOut<R, NP> o;
+  trial_eval<R, NP>(T, xx, o);
+  if (psi) psi[i] = o.psi;
+  if (lnpsi) lnpsi[i] = o.lnpsi;
+  if (d1)
+#pragma unroll
+    for (int p = 0; p < NP; ++p) d1[i * NP + p] = o.d1[p];
+  if (d2)
+#pragma unroll
+    for (int p = 0; p < NP; ++p) d2[i * NP + p] = o.d2[p];
+}
+
+template <typename R, int NP>
+__global__ void k_trial_vjp(TrialParams T, const R* x, long long n, const R* go, R* gx) {
+  const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  R xx[NP];
+#pragma unroll
+  for (int p = 0; p < NP; ++p) xx[p] = x[i * NP + p];
+  TrialOut<R, NP> o;
+  trial_eval<R, NP>(T, xx, o);
+  const R s = go[i];
+#pragma unroll
+  for (int p = 0; p < NP; ++p) gx[i * NP + p] = s * o.d1[p];
+}
+
+unsigned nblk(long long n, int bs = 256) { return (unsigned)((n + bs - 1) / bs); }
+
+}  // namespace
+
+// ------------------------------------------------------------------ simulation state
+struct pvd_sim {
+  pvd_config cfg;
+  int np, M, d, prec, cont, is;
+  long long cap;
+  void* coords[2];
+  int* who[2];
+  bool who_on;
+  double* w;
+  double* E;
+  double* ke;
+  unsigned long long* kmask;
+  Ctrl* ctrl;
+  uint64_t* flags;
+  TileRec* rec;
+  double* tr_vref;
+  double* tr_pop;
+  double* tr_dteff;
+  int* tr_births;
+  int* tr_deaths;
+  long long tmax;
+  Piece* pend;
+  Piece* fin;
+  int* klist;
+  unsigned long long* killtag;
+  double* rstage;
+  double* stage;
+  void* parent;
+  double* parent_w;
+  int n_parent;
+  double* desc;
+  int* cnt;
+  unsigned long long* dacc;
+  long long next_step;
+  unsigned epoch_next;
+  int status_seen;
+};
+
+namespace {
+
+size_t esize(const pvd_sim* s) { return s->prec == PVD_FP64 ? 8 : 4; }
+
+// buffer holding the current walkers (discrete: by step parity; continuous: in place)
+int cur_buf(const pvd_sim* s) { return s->cont ? 0 : (int)(s->next_step & 1); }
+
+template <typename R>
+StepArgs<R> make_args(pvd_sim* s) {
+  StepArgs<R> A;
+  std::memset(&A, 0, sizeof(A));
+  const int par = (int)(s->next_step & 1);
+  const int cb = cur_buf(s);
+  A.cin = (const R*)s->coords[cb];
+  A.cout = (R*)(s->cont ? s->coords[0] : s->coords[cb ^ 1]);
+  A.cap = s->cap;
+  A.np = s->np;
+  if (s->who_on) {
+    A.who_in = s->who[cb];
+    A.who_out = s->cont ? s->who[0] : s->who[cb ^ 1];
+  }
+  A.w = s->w;
+  A.E = s->E;
+  A.ke = s->ke;
+  A.kmask = s->kmask;
+  A.ctrl = s->ctrl;
+  A.flags = s->flags;
+  A.rec = s->rec;
+  A.tr_vref = s->tr_vref;
+  A.tr_pop = s->tr_pop;
+  A.tr_births = s->tr_births;
+  A.tr_deaths = s->tr_deaths;
+  A.tr_dteff = s->tr_dteff;
+  A.tmax = s->tmax;
+  A.pend = s->pend;
+  A.fin = s->fin;
+  A.klist = s->klist;
+  A.killtag = s->killtag;
+  A.rstage = s->rstage;
+  std::memcpy(&A.pes, &s->cfg.pes, sizeof(PesParams));
+  std::memcpy(&A.trial, &s->cfg.trial, sizeof(TrialParams));
+  for (int p = 0; p < MAXNP; ++p) {
+    A.sigma[p] = s->cfg.sigma[p];
+    A.inv_m[p] = s->cfg.inv_mass[p];
+  }
+  A.dt = s->cfg.delta_t;
+  A.alpha = s->cfg.alpha;
+  A.n0 = (double)s->cfg.num_walkers;
+  A.pop_lo = (double)s->cfg.num_walkers - (double)s->cfg.num_walkers * 0.5;
+  A.pop_hi = (double)s->cfg.num_walkers + (double)s->cfg.num_walkers * 0.5;
+  A.thresh_lo = s->cfg.thresh_lower;
+  A.rng.k0 = (uint32_t)s->cfg.seed;
+  A.rng.k1 = (uint32_t)(s->cfg.seed >> 32);
+  A.par = par;
+  const int be = s->cfg.branch_every > 0 ? s->cfg.branch_every : 1;
+  A.branch = (s->next_step % be) == 0 ? 1 : 0;
+  A.ms = s->M;
+  A.ds = s->d;
+  return A;
+}
+
+// ---- launch helpers, instantiated over the component count ----------------------------
+#define PVD_NP_SWITCH(np, ...)                                  \
+  switch (np) {                                                 \
+    case 1: { constexpr int NP = 1; __VA_ARGS__; } break;       \
+    case 3: { constexpr int NP = 3; __VA_ARGS__; } break;       \
+    case 9: { constexpr int NP = 9; __VA_ARGS__; } break;       \
+    case 18: { constexpr int NP = 18; __VA_ARGS__; } break;     \
+    default: return fail(E_BAD, "fused kernels support 1, 3, 9 or 18 components per walker"); \
+  }
+
+template <typename R, bool INJ>
+int launch_fused_step(pvd_sim* s, const StepArgs<R>& A, hipStream_t st) {
+  const unsigned g = nblk(s->cap, TILE);
+  const int pk = s->cfg.pes.kind;
+  if (pk == PES_EXTERNAL) return fail(E_BAD, "fused step needs a device potential");
+  if (!s->cont) {
+    if (s->is) {
+      PVD_NP_SWITCH(s->np, hipLaunchKernelGGL((k_accept<R, NP, INJ>), dim3(g), dim3(TILE), 0, st, A);
+                            hipLaunchKernelGGL((k_discrete<R, NP, INJ, true, false>), dim3(g), dim3(TILE), 0, st, A));
+    } else {
+      PVD_NP_SWITCH(s->np, hipLaunchKernelGGL((k_discrete<R, NP, INJ, false, false>), dim3(g), dim3(TILE), 0, st, A));
+    }
+  } else {
+    if (s->is) {
+      PVD_NP_SWITCH(s->np, hipLaunchKernelGGL((k_accept<R, NP, INJ>), dim3(g), dim3(TILE), 0, st, A);
+                            hipLaunchKernelGGL((k_cont<R, NP, INJ, true, false>), dim3(g), dim3(TILE), 0, st, A));
+    } else {
+      PVD_NP_SWITCH(s->np, hipLaunchKernelGGL((k_cont<R, NP, INJ, false, false>), dim3(g), dim3(TILE), 0, st, A));
+    }
+    hipLaunchKernelGGL((k_cont_replay<R>), dim3(1), dim3(RP_NT), 0, st, A);
+  }
+  HIPCHK(hipGetLastError());
+  return 0;
+}
+
+template <typename R>
+int run_steps(pvd_sim* s, int n_steps, const double* disp, const double* um, const double* ub,
+              hipStream_t st) {
+  for (int k = 0; k < n_steps; ++k) {
+    StepArgs<R> A = make_args<R>(s);
+    int rc;
+    if (disp) {
+      A.inj_disp = disp;
+      A.inj_umet = um;
+      A.inj_ubr = ub;
+      rc = launch_fused_step<R, true>(s, A, st);
+    } else {
+      rc = launch_fused_step<R, false>(s, A, st);
+    }
+    if (rc) return rc;
+    s->next_step += 1;
+    s->epoch_next += 1;
+  }
+  return 0;
+}
+
+template <typename R>
+int do_move(pvd_sim* s, const double* disp, const double* um, hipStream_t st) {
+  StepArgs<R> A = make_args<R>(s);
+  // the move happens in place in the current buffer
+  A.cout = (R*)A.cin;
+  A.inj_disp = disp;
+  A.inj_umet = um;
+  const unsigned g = nblk(s->cap, TILE);
+  if (s->is) {
+    if (disp) {
+      PVD_NP_SWITCH(s->np, hipLaunchKernelGGL((k_move<R, NP, true, true>), dim3(g), dim3(TILE), 0, st, A));
+    } else {
+      PVD_NP_SWITCH(s->np, hipLaunchKernelGGL((k_move<R, NP, false, true>), dim3(g), dim3(TILE), 0, st, A));
+    }
+  } else {
+    if (disp) hipLaunchKernelGGL((k_move<R, 1, true, false>), dim3(g), dim3(TILE), 0, st, A);
+    else hipLaunchKernelGGL((k_move<R, 1, false, false>), dim3(g), dim3(TILE), 0, st, A);
+  }
+  HIPCHK(hipGetLastError());
+  return 0;
+}
+
+template <typename R>
+int do_branch_ext(pvd_sim* s, const double* v, const double* ub, hipStream_t st) {
+  StepArgs<R> A = make_args<R>(s);
+  A.V_ext = v;
+  A.inj_ubr = ub;
+  const unsigned g = nblk(s->cap, TILE);
+  if (!s->cont) {
+    if (ub) {
+      if (s->is) hipLaunchKernelGGL((k_discrete<R, 1, true, true, true>), dim3(g), dim3(TILE), 0, st, A);
+      else hipLaunchKernelGGL((k_discrete<R, 1, true, false, true>), dim3(g), dim3(TILE), 0, st, A);
+    } else {
+      if (s->is) hipLaunchKernelGGL((k_discrete<R, 1, false, true, true>), dim3(g), dim3(TILE), 0, st, A);
+      else hipLaunchKernelGGL((k_discrete<R, 1, false, false, true>), dim3(g), dim3(TILE), 0, st, A);
+    }
+  } else {
+    if (s->is) hipLaunchKernelGGL((k_cont<R, 1, false, true, true>), dim3(g), dim3(TILE), 0, st, A);
+    else hipLaunchKernelGGL((k_cont<R, 1, false, false, true>), dim3(g), dim3(TILE), 0, st, A);
+    hipLaunchKernelGGL((k_cont_replay<R>), dim3(1), dim3(RP_NT), 0, st, A);
+  }
+  HIPCHK(hipGetLastError());
+  s->next_step += 1;
+  s->epoch_next += 1;
+  return 0;
+}
+
+template <typename R>
+int do_start(pvd_sim* s, const double* v, hipStream_t st) {
+  StepArgs<R> A = make_args<R>(s);
+  A.V_ext = v;
+  const unsigned g = nblk(s->cap, TILE);
+  if (v) {
+    if (s->is) {
+      PVD_NP_SWITCH(s->np, hipLaunchKernelGGL((k_start<R, NP, true, true>), dim3(g), dim3(TILE), 0, st, A, s->cont));
+    } else {
+      hipLaunchKernelGGL((k_start<R, 1, false, true>), dim3(g), dim3(TILE), 0, st, A, s->cont);
+    }
+  } else {
+    if (s->is) {
+      PVD_NP_SWITCH(s->np, hipLaunchKernelGGL((k_start<R, NP, true, false>), dim3(g), dim3(TILE), 0, st, A, s->cont));
+    } else {
+      PVD_NP_SWITCH(s->np, hipLaunchKernelGGL((k_start<R, NP, false, false>), dim3(g), dim3(TILE), 0, st, A, s->cont));
+    }
+  }
+  HIPCHK(hipGetLastError());
+  return 0;
+}
+
+int read_ctrl(pvd_sim* s, Ctrl& c, hipStream_t st) {
+  HIPCHK(hipMemcpyAsync(&c, s->ctrl, sizeof(Ctrl), hipMemcpyDeviceToHost, st));
+  HIPCHK(hipStreamSynchronize(st));
+  return 0;
+}
+
+template <typename R>
+int pes_eval_t(const pvd_pes_desc* pes, int np, const void* x, long long n, void* v, void* g_out,
+               void* gx, hipStream_t st) {
+  PesParams P;
+  std::memcpy(&P, pes, sizeof(P));
+  if (g_out) {
+    PVD_NP_SWITCH(np, hipLaunchKernelGGL((k_pes_vjp<R, NP>), dim3(nblk(n)), dim3(256), 0, st, P,
+                                         (const R*)x, n, (const R*)g_out, (R*)gx));
+  } else {
+    PVD_NP_SWITCH(np, hipLaunchKernelGGL((k_pes_eval<R, NP>), dim3(nblk(n)), dim3(256), 0, st, P,
+                                         (const R*)x, n, (R*)v));
+  }
+  HIPCHK(hipGetLastError());
+  return 0;
+}
+
+template <typename R>
+int trial_eval_t(const pvd_trial_desc* tr, int np, const void* x, long long n, void* psi, void* lnpsi,
+                 void* d1, void* d2, const void* g_out, void* gx, hipStream_t st) {
+  TrialParams T;
+  std::memcpy(&T, tr, sizeof(T));
+  if (g_out) {
+    PVD_NP_SWITCH(np, hipLaunchKernelGGL((k_trial_vjp<R, NP>), dim3(nblk(n)), dim3(256), 0, st, T,
+                                         (const R*)x, n, (const R*)g_out, (R*)gx));
+  } else {
+    PVD_NP_SWITCH(np, hipLaunchKernelGGL((k_trial_eval<R, NP>), dim3(nblk(n)), dim3(256), 0, st, T,
+                                         (const R*)x, n, (R*)psi, (R*)lnpsi, (R*)d1, (R*)d2));
+  }
+  HIPCHK(hipGetLastError());
+  return 0;
+}
+
+void free_sim(pvd_sim* s) {
+  if (!s) return;
+  void* ptrs[] = {s->coords[0], s->coords[1], s->who[0], s->who[1], s->w, s->E, s->ke, s->kmask,
+                  s->ctrl, s->flags, s->rec, s->tr_vref, s->tr_pop, s->tr_dteff, s->tr_births,
+                  s->tr_deaths, s->pend, s->fin, s->klist, s->killtag, s->rstage, s->stage, s->parent, s->parent_w, s->desc,
+                  s->cnt, s->dacc};
+  for (void* p : ptrs)
+    if (p) (void)hipFree(p);
+  delete s;
+}
+
+}  // namespace
+
+// ===================================================================== C entry points
+extern "C" {
+
+const char* pvd_version(void) { return "pyvibdmc_amd 0.1 (gfx950)"; }
+
+const char* pvd_last_error(void) { return g_err.c_str(); }
+
+int pvd_sim_create(const pvd_config* cfg, pvd_sim** out) {
+  if (!cfg || !out) return fail(E_BAD, "null argument");
+  *out = nullptr;
+  const int np = cfg->n_atoms * cfg->n_dim;
+  if (cfg->num_walkers <= 0 || cfg->n_atoms <= 0 || cfg->n_dim <= 0 || np > PVD_MAX_COMP)
+    return fail(E_BAD, "bad walker shape");
+  if (cfg->precision != PVD_FP64 && cfg->precision != PVD_FP32) return fail(E_BAD, "bad precision");
+  if (cfg->weighting != PVD_DISCRETE && cfg->weighting != PVD_CONTINUOUS)
+    return fail(E_BAD, "bad weighting");
+  if (cfg->trial.kind != TRIAL_NONE && np > PVD_FUSED_MAX_COMP)
+    return fail(E_BAD, "importance sampling supports at most 18 components per walker");
+  pvd_sim* s = new pvd_sim();
+  std::memset((void*)s, 0, sizeof(pvd_sim));
+  s->cfg = *cfg;
+  s->np = np;
+  s->M = cfg->n_atoms;
+  s->d = cfg->n_dim;
+  s->prec = cfg->precision;
+  s->cont = cfg->weighting == PVD_CONTINUOUS;
+  s->is = cfg->trial.kind != TRIAL_NONE;
+  long long cap = cfg->capacity;
+  const long long need = (long long)std::floor(1.5 * cfg->num_walkers) + 1;
+  if (cap <= 0) cap = need;
+  cap = ((cap + TILE - 1) / TILE) * TILE;
+  if (cap >= (1ll << 30)) {
+    delete s;
+    return fail(E_BAD, "capacity must stay below 2^30 walkers");
+  }
+  s->cap = cap;
+  s->tmax = cfg->num_timesteps > 0 ? cfg->num_timesteps : 1;
+  const long long ntiles = cap / TILE;
+  const size_t es = esize(s);
+#define ALLOC(ptr, bytes)                                    \
+  do {                                                       \
+    hipError_t e_ = hipMalloc((void**)&(ptr), (bytes));      \
+    if (e_ != hipSuccess) {                                  \
+      free_sim(s);                                           \
+      return fail(E_HIP, std::string("hipMalloc: ") + hipGetErrorString(e_)); \
+    }                                                        \
+  } while (0)
+  ALLOC(s->coords[0], es * np * cap);
+  ALLOC(s->coords[1], es * np * cap);
+  ALLOC(s->who[0], sizeof(int) * cap);
+  ALLOC(s->who[1], sizeof(int) * cap);
+  ALLOC(s->w, sizeof(double) * cap);
+  ALLOC(s->E, sizeof(double) * cap);
+  ALLOC(s->ke, sizeof(double) * cap);
+  ALLOC(s->kmask, sizeof(unsigned long long) * ntiles * (TILE / PVD_WAVE));
+  ALLOC(s->ctrl, sizeof(Ctrl));
+  ALLOC(s->flags, sizeof(uint64_t) * ntiles);
+  ALLOC(s->rec, sizeof(TileRec) * ntiles);
+  ALLOC(s->tr_vref, sizeof(double) * s->tmax);
+  ALLOC(s->tr_pop, sizeof(double) * s->tmax);
+  ALLOC(s->tr_dteff, sizeof(double) * s->tmax);
+  ALLOC(s->tr_births, sizeof(int) * s->tmax);
+  ALLOC(s->tr_deaths, sizeof(int) * s->tmax);
+  if (s->cont) {
+    ALLOC(s->pend, sizeof(Piece) * RP_PCAP * 2);
+    ALLOC(s->fin, sizeof(Piece) * RP_FCAP);
+    ALLOC(s->klist, sizeof(int) * RP_KCAP);
+    ALLOC(s->killtag, sizeof(unsigned long long) * cap);
+    ALLOC(s->rstage, sizeof(double) * (size_t)RP_FCAP * (np + 2));
+  }
+  ALLOC(s->stage, sizeof(double) * np * cap);
+#undef ALLOC
+  hipError_t e = hipMemset(s->flags, 0, sizeof(uint64_t) * ntiles);
+  if (e == hipSuccess) e = hipMemset(s->ctrl, 0, sizeof(Ctrl));
+  if (e == hipSuccess) e = hipMemset(s->tr_vref, 0, sizeof(double) * s->tmax);
+  if (e == hipSuccess) e = hipMemset(s->tr_pop, 0, sizeof(double) * s->tmax);
+  if (e == hipSuccess) e = hipMemset(s->tr_dteff, 0, sizeof(double) * s->tmax);
+  if (e == hipSuccess) e = hipMemset(s->tr_births, 0, sizeof(int) * s->tmax);
+  if (e == hipSuccess) e = hipMemset(s->tr_deaths, 0, sizeof(int) * s->tmax);
+  if (e == hipSuccess && s->killtag) e = hipMemset(s->killtag, 0, sizeof(unsigned long long) * cap);
+  if (e == hipSuccess) e = hipDeviceSynchronize();
+  if (e != hipSuccess) {
+    free_sim(s);
+    return fail(E_HIP, std::string("init: ") + hipGetErrorString(e));
+  }
+  s->epoch_next = 1;
+  *out = s;
+  return 0;
+}
+
+int pvd_sim_destroy(pvd_sim* sim) {
+  if (!sim) return 0;
+  (void)hipDeviceSynchronize();
+  free_sim(sim);
+  return 0;
+}
+
+int pvd_sim_set_walkers(pvd_sim* s, const double* coords, int32_t n, const double* weights,
+                        int64_t step0, void* stream) {
+  if (!s || !coords) return fail(E_BAD, "null argument");
+  if (n <= 0 || n > s->cap) return fail(E_BAD, "walker count outside [1, capacity]");
+  if (step0 < 0) return fail(E_BAD, "negative step");
+  hipStream_t st = (hipStream_t)stream;
+  s->next_step = step0;
+  const int cb = cur_buf(s);
+  HIPCHK(hipMemcpyAsync(s->stage, coords, sizeof(double) * (size_t)n * s->np, hipMemcpyHostToDevice, st));
+  if (s->prec == PVD_FP64)
+    hipLaunchKernelGGL((k_aos_to_planes<double>), dim3(nblk(n)), dim3(256), 0, st, s->stage,
+                       (double*)s->coords[cb], (long long)n, s->np, s->cap);
+  else
+    hipLaunchKernelGGL((k_aos_to_planes<float>), dim3(nblk(n)), dim3(256), 0, st, s->stage,
+                       (float*)s->coords[cb], (long long)n, s->np, s->cap);
+  HIPCHK(hipGetLastError());
+  if (s->cont) {
+    if (weights) {
+      HIPCHK(hipMemcpyAsync(s->w, weights, sizeof(double) * n, hipMemcpyHostToDevice, st));
+    } else {
+      return fail(E_BAD, "continuous weighting needs weights");
+    }
+  }
+  Ctrl c;
+  std::memset(&c, 0, sizeof(c));
+  const int par = (int)(step0 & 1);
+  c.n[par] = n;
+  c.step[par] = (unsigned long long)step0;
+  s->epoch_next += 2;
+  c.epoch[par] = s->epoch_next;
+  HIPCHK(hipMemcpyAsync(s->ctrl, &c, sizeof(c), hipMemcpyHostToDevice, st));
+  HIPCHK(hipStreamSynchronize(st));
+  s->who_on = false;
+  return 0;
+}
+
+int pvd_sim_get_walkers(pvd_sim* s, double* coords, double* weights, int64_t max_walkers, int32_t* n_out,
+                        void* stream) {
+  if (!s || !coords) return fail(E_BAD, "null argument");
+  hipStream_t st = (hipStream_t)stream;
+  Ctrl c;
+  int rc = read_ctrl(s, c, st);
+  if (rc) return rc;
+  long long step = (long long)s->next_step;
+  if (c.status != ST_OK) step = c.status_step;
+  const int par = (int)(step & 1);
+  const int n = c.n[par];
+  if (n > max_walkers) return fail(E_BAD, "output buffer too small");
+  const int cb = s->cont ? 0 : par;
+  if (s->prec == PVD_FP64)
+    hipLaunchKernelGGL((k_planes_to_aos<double>), dim3(nblk(n)), dim3(256), 0, st,
+                       (const double*)s->coords[cb], s->stage, (long long)n, s->np, s->cap);
+  else
+    hipLaunchKernelGGL((k_planes_to_aos<float>), dim3(nblk(n)), dim3(256), 0, st,
+                       (const float*)s->coords[cb], s->stage, (long long)n, s->np, s->cap);
+  HIPCHK(hipGetLastError());
+  HIPCHK(hipMemcpyAsync(coords, s->stage, sizeof(double) * (size_t)n * s->np, hipMemcpyDeviceToHost, st));
+  if (weights && s->cont)
+    HIPCHK(hipMemcpyAsync(weights, s->w, sizeof(double) * n, hipMemcpyDeviceToHost, st));
+  HIPCHK(hipStreamSynchronize(st));
+  if (n_out) *n_out = n;
+  return 0;
+}
+
+int pvd_sim_start(pvd_sim* s, void* stream) {
+  if (!s) return fail(E_BAD, "null argument");
+  if (s->cfg.pes.kind == PES_EXTERNAL) return fail(E_BAD, "external potential: use pvd_sim_start_external");
+  return s->prec == PVD_FP64 ? do_start<double>(s, nullptr, (hipStream_t)stream)
+                             : do_start<float>(s, nullptr, (hipStream_t)stream);
+}
+
+int pvd_sim_start_external(pvd_sim* s, const double* v, void* stream) {
+  if (!s || !v) return fail(E_BAD, "null argument");
+  return s->prec == PVD_FP64 ? do_start<double>(s, v, (hipStream_t)stream)
+                             : do_start<float>(s, v, (hipStream_t)stream);
+}
+
+int pvd_sim_run(pvd_sim* s, int32_t n_steps, void* stream) {
+  if (!s) return fail(E_BAD, "null argument");
+  if (n_steps < 0) return fail(E_BAD, "negative step count");
+  return s->prec == PVD_FP64 ? run_steps<double>(s, n_steps, nullptr, nullptr, nullptr, (hipStream_t)stream)
+                             : run_steps<float>(s, n_steps, nullptr, nullptr, nullptr, (hipStream_t)stream);
+}
+
+int pvd_sim_step_injected(pvd_sim* s, const double* disp, const double* um, const double* ub, void* stream) {
+  if (!s || !disp) return fail(E_BAD, "null argument");
+  if (s->is && !um) return fail(E_BAD, "importance sampling needs Metropolis uniforms");
+  const int be = s->cfg.branch_every > 0 ? s->cfg.branch_every : 1;
+  if (!s->cont && (s->next_step % be) == 0 && !ub) return fail(E_BAD, "branching step needs uniforms");
+  return s->prec == PVD_FP64 ? run_steps<double>(s, 1, disp, um, ub, (hipStream_t)stream)
+                             : run_steps<float>(s, 1, disp, um, ub, (hipStream_t)stream);
+}
+
+int pvd_sim_move(pvd_sim* s, const double* disp, const double* um, void* stream) {
+  if (!s) return fail(E_BAD, "null argument");
+  if (s->is && disp && !um) return fail(E_BAD, "importance sampling needs Metropolis uniforms");
+  return s->prec == PVD_FP64 ? do_move<double>(s, disp, um, (hipStream_t)stream)
+                             : do_move<float>(s, disp, um, (hipStream_t)stream);
+}
+
+int pvd_sim_branch_external(pvd_sim* s, const double* v, const double* ub, void* stream) {
+  if (!s || !v) return fail(E_BAD, "null argument");
+  return s->prec == PVD_FP64 ? do_branch_ext<double>(s, v, ub, (hipStream_t)stream)
+                             : do_branch_ext<float>(s, v, ub, (hipStream_t)stream);
+}
+
+int pvd_sim_planes(pvd_sim* s, void** planes, int64_t* stride, int32_t* n, void* stream) {
+  if (!s) return fail(E_BAD, "null argument");
+  Ctrl c;
+  int rc = read_ctrl(s, c, (hipStream_t)stream);
+  if (rc) return rc;
+  const int cb = cur_buf(s);
+  if (planes) *planes = s->coords[cb];
+  if (stride) *stride = s->cap;
+  if (n) *n = c.n[(int)(s->next_step & 1)];
+  return 0;
+}
+
+int pvd_sim_status(pvd_sim* s, int32_t* status, int64_t* fail_step, int32_t* n, double* vref,
+                   int64_t* next_step, void* stream) {
+  if (!s) return fail(E_BAD, "null argument");
+  Ctrl c;
+  int rc = read_ctrl(s, c, (hipStream_t)stream);
+  if (rc) return rc;
+  const long long step = c.status != ST_OK ? c.status_step : (long long)s->next_step;
+  const int par = (int)(step & 1);
+  if (status) *status = c.status;
+  if (fail_step) *fail_step = c.status != ST_OK ? c.status_step : -1;
+  if (n) *n = c.n[par];
+  if (vref) *vref = c.vref[par];
+  if (next_step) *next_step = step;
+  if (c.status != ST_OK) return fail(c.status, status_text(c.status));
+  return 0;
+}
+
+int pvd_sim_trajectory(pvd_sim* s, int64_t t0, int32_t count, double* vref, double* pop, int32_t* births,
+                       int32_t* deaths, double* eff_t) {
+  if (!s) return fail(E_BAD, "null argument");
+  if (t0 < 0 || count < 0 || t0 + count > s->tmax) return fail(E_BAD, "trajectory range");
+  HIPCHK(hipDeviceSynchronize());
+  if (vref) HIPCHK(hipMemcpy(vref, s->tr_vref + t0, sizeof(double) * count, hipMemcpyDeviceToHost));
+  if (pop) HIPCHK(hipMemcpy(pop, s->tr_pop + t0, sizeof(double) * count, hipMemcpyDeviceToHost));
+  if (births) HIPCHK(hipMemcpy(births, s->tr_births + t0, sizeof(int) * count, hipMemcpyDeviceToHost));
+  if (deaths) HIPCHK(hipMemcpy(deaths, s->tr_deaths + t0, sizeof(int) * count, hipMemcpyDeviceToHost));
+  if (eff_t) HIPCHK(hipMemcpy(eff_t, s->tr_dteff + t0, sizeof(double) * count, hipMemcpyDeviceToHost));
+  return 0;
+}
+
+int pvd_sim_desc_begin(pvd_sim* s, void* stream) {
+  if (!s) return fail(E_BAD, "null argument");
+  hipStream_t st = (hipStream_t)stream;
+  Ctrl c;
+  int rc = read_ctrl(s, c, st);
+  if (rc) return rc;
+  const int n = c.n[(int)(s->next_step & 1)];
+  const int cb = cur_buf(s);
+  const size_t es = esize(s);
+  if (!s->parent) {
+    HIPCHK(hipMalloc(&s->parent, es * s->np * s->cap));
+    HIPCHK(hipMalloc((void**)&s->parent_w, sizeof(double) * s->cap));
+    HIPCHK(hipMalloc((void**)&s->desc, sizeof(double) * s->cap));
+    HIPCHK(hipMalloc((void**)&s->cnt, sizeof(int) * s->cap));
+    HIPCHK(hipMalloc((void**)&s->dacc, sizeof(unsigned long long) * s->cap));
+  }
+  hipLaunchKernelGGL(k_iota, dim3(nblk(n)), dim3(256), 0, st, s->who[cb], (long long)n);
+  if (s->prec == PVD_FP64)
+    hipLaunchKernelGGL((k_copy_planes<double>), dim3(nblk(n)), dim3(256), 0, st, (const double*)s->coords[cb],
+                       (double*)s->parent, (long long)n, s->np, s->cap);
+  else
+    hipLaunchKernelGGL((k_copy_planes<float>), dim3(nblk(n)), dim3(256), 0, st, (const float*)s->coords[cb],
+                       (float*)s->parent, (long long)n, s->np, s->cap);
+  if (s->cont) HIPCHK(hipMemcpyAsync(s->parent_w, s->w, sizeof(double) * n, hipMemcpyDeviceToDevice, st));
+  HIPCHK(hipGetLastError());
+  s->n_parent = n;
+  s->who_on = true;
+  return 0;
+}
+
+int pvd_sim_desc_end(pvd_sim* s, double* desc_wts, double* parent, double* parent_wts, int32_t* n_parent,
+                     void* stream) {
+  if (!s) return fail(E_BAD, "null argument");
+  if (!s->who_on) return fail(E_BAD, "no descendant-weighting window is open");
+  hipStream_t st = (hipStream_t)stream;
+  Ctrl c;
+  int rc = read_ctrl(s, c, st);
+  if (rc) return rc;
+  const int n = c.n[(int)(s->next_step & 1)];
+  const int cb = cur_buf(s);
+  const long long npar = s->n_parent;
+  HIPCHK(hipMemsetAsync(s->cnt, 0, sizeof(int) * npar, st));
+  HIPCHK(hipMemsetAsync(s->dacc, 0, sizeof(unsigned long long) * npar, st));
+  hipLaunchKernelGGL(k_desc_count, dim3(nblk(n)), dim3(256), 0, st, s->who[cb], (long long)n, s->cnt);
+  if (s->cont)
+    hipLaunchKernelGGL(k_desc_cont, dim3(nblk(n)), dim3(256), 0, st, s->who[cb], s->w, (long long)n, s->cnt,
+                       s->desc, s->dacc);
+  hipLaunchKernelGGL(k_desc_finish, dim3(nblk(npar)), dim3(256), 0, st, s->cnt, s->dacc, s->desc, npar, s->cont);
+  HIPCHK(hipGetLastError());
+  if (desc_wts) HIPCHK(hipMemcpyAsync(desc_wts, s->desc, sizeof(double) * npar, hipMemcpyDeviceToHost, st));
+  if (parent) {
+    if (s->prec == PVD_FP64)
+      hipLaunchKernelGGL((k_planes_to_aos<double>), dim3(nblk(npar)), dim3(256), 0, st, (const double*)s->parent,
+                         s->stage, npar, s->np, s->cap);
+    else
+      hipLaunchKernelGGL((k_planes_to_aos<float>), dim3(nblk(npar)), dim3(256), 0, st, (const float*)s->parent,
+                         s->stage, npar, s->np, s->cap);
+    HIPCHK(hipMemcpyAsync(parent, s->stage, sizeof(double) * npar * s->np, hipMemcpyDeviceToHost, st));
+  }
+  if (parent_wts && s->cont)
+    HIPCHK(hipMemcpyAsync(parent_wts, s->parent_w, sizeof(double) * npar, hipMemcpyDeviceToHost, st));
+  HIPCHK(hipStreamSynchronize(st));
+  if (n_parent) *n_parent = (int)npar;
+  s->who_on = false;
+  return 0;
+}
+
+int pvd_pes_eval(const pvd_pes_desc* pes, int32_t precision, int32_t n_atoms, int32_t n_dim, const void* coords,
+                 int64_t n, void* v_out, void* stream) {
+  if (!pes || !coords || !v_out) return fail(E_BAD, "null argument");
+  if (n <= 0) return 0;
+  const int np = n_atoms * n_dim;
+  return precision == PVD_FP64 ? pes_eval_t<double>(pes, np, coords, n, v_out, nullptr, nullptr, (hipStream_t)stream)
+                               : pes_eval_t<float>(pes, np, coords, n, v_out, nullptr, nullptr, (hipStream_t)stream);
+}
+
+int pvd_pes_vjp(const pvd_pes_desc* pes, int32_t precision, int32_t n_atoms, int32_t n_dim, const void* coords,
+                int64_t n, const void* grad_out, void* grad_coords, void* stream) {
+  if (!pes || !coords || !grad_out || !grad_coords) return fail(E_BAD, "null argument");
+  if (n <= 0) return 0;
+  const int np = n_atoms * n_dim;
+  return precision == PVD_FP64
+             ? pes_eval_t<double>(pes, np, coords, n, nullptr, (void*)grad_out, grad_coords, (hipStream_t)stream)
+             : pes_eval_t<float>(pes, np, coords, n, nullptr, (void*)grad_out, grad_coords, (hipStream_t)stream);
+}
+
+int pvd_trial_eval(const pvd_trial_desc* trial, int32_t precision, int32_t n_atoms, int32_t n_dim,
+                   const void* coords, int64_t n, void* psi, void* lnpsi, void* dlnpsi, void* d2, void* stream) {
+  if (!trial || !coords) return fail(E_BAD, "null argument");
+  if (n <= 0) return 0;
+  const int np = n_atoms * n_dim;
+  return precision == PVD_FP64 ? trial_eval_t<double>(trial, np, coords, n, psi, lnpsi, dlnpsi, d2, nullptr,
+                                                      nullptr, (hipStream_t)stream)
+                               : trial_eval_t<float>(trial, np, coords, n, psi, lnpsi, dlnpsi, d2, nullptr,
+                                                     nullptr, (hipStream_t)stream);
+}
+
+int pvd_trial_vjp(const pvd_trial_desc* trial, int32_t precision, int32_t n_atoms, int32_t n_dim,
+                  const void* coords, int64_t n, const void* grad_out, void* grad_coords, void* stream) {
+  if (!trial || !coords || !grad_out || !grad_coords) return fail(E_BAD, "null argument");
+  if (n <= 0) return 0;
+  const int np = n_atoms * n_dim;
+  return precision == PVD_FP64 ? trial_eval_t<double>(trial, np, coords, n, nullptr, nullptr, nullptr, nullptr,
+                                                      grad_out, grad_coords, (hipStream_t)stream)
+                               : trial_eval_t<float>(trial, np, coords, n, nullptr, nullptr, nullptr, nullptr,
+                                                     grad_out, grad_coords, (hipStream_t)stream);
+}
+
+}  // extern "C"

@@ -1,0 +1,1017 @@
+// DMC step kernels.
+//
+// Walkers live in HBM as structure-of-arrays planes: component p of walker slot i is at
+// coords[p * cap + i] (p = atom*d + c), double-buffered for the discrete branch.  One
+// step of the discrete algorithm (reference pyvibdmc.py:540-547, 786-793, 391-431,
+// 651-661) is ONE launch of k_discrete: each 256-walker tile moves its walkers (Philox
+// + Box-Muller, or injected draws), evaluates the PES in registers, forms the
+// branching weight and copy count, and writes its survivors to their final slots in
+// np.repeat order.  The output offset of a tile comes from a single-pass decoupled
+// look-back over per-tile flags; the tile that comes last in that order reduces the
+// per-tile partial sums in a fixed order and writes vref and the population for the
+// next step.  All control values are double-buffered by step parity `par`, which is
+// a launch argument, so a launch never reads a value that the same launch writes.
+#pragma once
+#include "pvd_models.hpp"
+
+namespace pvd {
+
+constexpr int TILE = 256;
+constexpr int MAXNP = 96;  // components per walker on the runtime-count paths
+
+enum Status : int {
+  ST_OK = 0,
+  ST_POP = 1,          // population outside [0.5, 1.5] N0       (pyvibdmc.py:409-413, 714-717)
+  ST_WEIGHT = 3,       // non-finite or huge branching weight    (pyvibdmc.py:397-400)
+  ST_CAPACITY = 4,     // more walkers than slots
+  ST_BRANCH = 5,       // continuous split outside supported sizes
+  ST_SPIN = 6,         // a bounded device wait expired
+};
+
+struct Ctrl {
+  int n[2];
+  double vref[2];
+  unsigned long long step[2];
+  unsigned epoch[2];
+  unsigned ticket[2];
+  unsigned done[2];
+  int acc[2];
+  int kills[2];
+  double sumw[2];
+  double sumwe[2];
+  int status;
+  int pad0;
+  long long status_step;
+  double eff_t;
+  unsigned list_n;
+  unsigned pad1;
+};
+
+struct TileRec {
+  double s0;  // discrete: sum c*E | continuous: sum w*E
+  double s1;  // continuous: sum w | discrete: unused
+  double mx;  // continuous: max weight in the tile
+  int count;  // discrete: sum c   | continuous: kills
+  int births;
+  int deaths;
+  int bad;
+};
+
+struct Piece {
+  double v;
+  int slot;
+  int src;
+  int birth;  // kill index at which the piece was created (-1: an original walker)
+  int pad;
+};
+
+template <typename R>
+struct StepArgs {
+  const R* cin;
+  R* cout;
+  long long cap;
+  int np;              // components per walker (runtime copy of NP)
+  const int* who_in;   // descendant tracker (nullptr when no window is open)
+  int* who_out;
+  double* w;           // continuous weights
+  double* E;           // continuous: per-walker energy used for the weight
+  double* ke;          // split IS path: local kinetic energy of each walker
+  const double* V_ext; // split path: energies computed outside the kernel
+  unsigned long long* kmask;  // continuous: per-wave kill masks
+  Ctrl* ctrl;
+  uint64_t* flags;
+  TileRec* rec;
+  double* tr_vref;
+  double* tr_pop;
+  int* tr_births;
+  int* tr_deaths;
+  double* tr_dteff;
+  long long tmax;
+  const double* inj_disp;  // injected displacements (N, M, d) C order
+  const double* inj_umet;  // injected Metropolis uniforms
+  const double* inj_ubr;   // injected branching uniforms
+  Piece* pend;             // continuous split scratch
+  Piece* fin;
+  int* klist;              // kill list of the step (ascending slots)
+  unsigned long long* killtag;  // per slot: (step << 32) | kill index, valid for this step only
+  double* rstage;          // staged source walkers of the split copies
+  PesParams pes;
+  TrialParams trial;
+  double sigma[MAXNP];
+  double inv_m[MAXNP];
+  double dt, alpha, n0, pop_lo, pop_hi, thresh_lo;
+  Rng rng;
+  int par;
+  int branch;
+  int ms, ds;  // atoms, dims
+};
+
+// ---------------------------------------------------------------------------- helpers
+template <typename R, int NP>
+PVD_DEV void load_walker(const StepArgs<R>& A, long long i, R* x) {
+#pragma unroll
+  for (int p = 0; p < NP; ++p) x[p] = A.cin[p * A.cap + i];
+}
+
+// displacement draws for walker i: disp = sigma*z (Philox) or the injected values
+template <typename R, int NP, bool INJ>
+PVD_DEV void draw_disp(const StepArgs<R>& A, long long i, unsigned long long step, R* disp) {
+  if constexpr (INJ) {
+#pragma unroll
+    for (int p = 0; p < NP; ++p) disp[p] = (R)A.inj_disp[i * NP + p];
+  } else {
+    R z[NP];
+    Normals<R, NP>::draw(A.rng, (uint32_t)i, step, z);
+#pragma unroll
+    for (int p = 0; p < NP; ++p) disp[p] = (R)A.sigma[p] * z[p];
+  }
+}
+
+template <typename R, bool INJ>
+PVD_DEV double draw_u(const StepArgs<R>& A, long long i, unsigned long long step, uint32_t purpose) {
+  if constexpr (INJ) {
+    return purpose == RNG_BRANCH ? A.inj_ubr[i] : A.inj_umet[i];
+  } else {
+    return uniform_d(A.rng, (uint32_t)i, step, purpose);
+  }
+}
+
+// Local kinetic energy -0.5 sum_a sum_c inv_m * d2psi/psi, summed over atoms first then
+// over components (imp_samp.py:49-53).
+template <typename R, int NP>
+PVD_DEV R local_kin(const StepArgs<R>& A, const R* sec) {
+  R tot = R(0);
+  for (int c = 0; c < A.ds; ++c) {
+    R s = R(0);
+    for (int a = 0; a < A.ms; ++a) {
+      const int p = a * A.ds + c;
+      R v = R(0);
+#pragma unroll
+      for (int q = 0; q < NP; ++q)
+        if (q == p) v = (R)A.inv_m[q] * sec[q];
+      s = (a == 0) ? v : s + v;
+    }
+    tot = (c == 0) ? s : tot + s;
+  }
+  return R(-0.5) * tot;
+}
+
+// Importance-sampled move of one walker (pyvibdmc.py:549-612, imp_samp.py:28-47).
+// On return x holds the new position and the function returns the local energy's
+// kinetic part at that position; `acc` tells whether the proposal was accepted.
+template <typename R, int NP, bool INJ>
+PVD_DEV R is_move(const StepArgs<R>& A, long long i, unsigned long long step, R* x, bool& acc) {
+  TrialOut<R, NP> tx, ty;
+  trial_eval<R, NP>(A.trial, x, tx);
+  R disp[NP], y[NP], Dx[NP], Dy[NP];
+  draw_disp<R, NP, INJ>(A, i, step, disp);
+  const R dt = (R)A.dt;
+#pragma unroll
+  for (int p = 0; p < NP; ++p) {
+    Dx[p] = (R)A.inv_m[p] * tx.d1[p];
+    y[p] = (x[p] + disp[p]) + Dx[p] * dt;
+  }
+  trial_eval<R, NP>(A.trial, y, ty);
+#pragma unroll
+  for (int p = 0; p < NP; ++p) Dy[p] = (R)A.inv_m[p] * ty.d1[p];
+  R met;
+  if constexpr (sizeof(R) == 8) {
+    // literal form of imp_samp.py:37-46
+    const R r = ty.psi / tx.psi;
+    const R ratio = r * r;
+    R tc[NP];
+#pragma unroll
+    for (int p = 0; p < NP; ++p) {
+      const R s2 = (R)A.sigma[p] * (R)A.sigma[p];
+      const R a1 = (x[p] - y[p]) - Dy[p] * dt;
+      const R a2 = (y[p] - x[p]) - Dx[p] * dt;
+      const R t1 = pvd_exp<R>((R(-1) * (a1 * a1)) / (R(2) * s2));
+      const R t2 = pvd_exp<R>((R(-1) * (a2 * a2)) / (R(2) * s2));
+      tc[p] = t1 / t2;
+    }
+    if (A.ds == 1 && NP == 1) {
+      met = tc[0] * ratio;
+    } else {
+      R prod = R(1);
+      for (int c = 0; c < A.ds; ++c) {
+        R pc = R(1);
+        for (int a = 0; a < A.ms; ++a) {
+          const int p = a * A.ds + c;
+          R v = R(1);
+#pragma unroll
+          for (int q = 0; q < NP; ++q)
+            if (q == p) v = tc[q];
+          pc = (a == 0) ? v : pc * v;
+        }
+        prod = (c == 0) ? pc : prod * pc;
+      }
+      met = prod * ratio;
+    }
+    if (tx.psi * ty.psi <= R(0)) met = R(0);
+  } else {
+    // log form (same mathematics, no overflow in single precision)
+    R la = R(2) * (ty.lnpsi - tx.lnpsi);
+#pragma unroll
+    for (int p = 0; p < NP; ++p) {
+      const R inv2s2 = R(0.5) / ((R)A.sigma[p] * (R)A.sigma[p]);
+      const R a1 = (x[p] - y[p]) - Dy[p] * dt;
+      const R a2 = (y[p] - x[p]) - Dx[p] * dt;
+      la += (a2 * a2 - a1 * a1) * inv2s2;
+    }
+    met = pvd_exp<R>(la);
+  }
+  const double u = draw_u<R, INJ>(A, i, step, RNG_METRO);
+  acc = (double)met > u;
+  if (acc) {
+#pragma unroll
+    for (int p = 0; p < NP; ++p) x[p] = y[p];
+  }
+  return local_kin<R, NP>(A, acc ? ty.d2 : tx.d2);
+}
+
+// pack / unpack a look-back flag: epoch (32) | status (2) | value (30)
+PVD_DEV uint64_t pack_flag(unsigned epoch, unsigned st, unsigned v) {
+  return ((uint64_t)epoch << 32) | ((uint64_t)st << 30) | (uint64_t)(v & 0x3FFFFFFFu);
+}
+enum : unsigned { FL_AGG = 1u, FL_INC = 2u };
+
+// Exclusive prefix of tile `tile` from the flags of its predecessors.  Called by all
+// lanes of wave 0.  Predecessors obtained their tickets earlier and are resident, so
+// every wait ends; the spin is still bounded and reports ST_SPIN if it ever expires.
+PVD_DEV int look_back(uint64_t* flags, int tile, unsigned epoch, Ctrl* C, bool& spun_out) {
+  const int l = lane_id();
+  int prefix = 0;
+  int j = tile - 1;
+  spun_out = false;
+  while (true) {
+    const int t = j - l;
+    uint64_t f = 0;
+    unsigned st = FL_INC, val = 0;
+    if (t >= 0) {
+      long long spins = 0;
+      while (true) {
+        f = ld_acquire(&flags[t]);
+        if ((unsigned)(f >> 32) == epoch && ((f >> 30) & 3u) != 0u) break;
+        if (++spins > (1ll << 26)) {
+          spun_out = true;
+          break;
+        }
+        __builtin_amdgcn_s_sleep(1);
+      }
+      st = (unsigned)((f >> 30) & 3u);
+      val = (unsigned)(f & 0x3FFFFFFFu);
+    }
+    const bool anyspin = __any(spun_out);
+    if (anyspin) {
+      spun_out = true;
+      return 0;
+    }
+    const unsigned long long inc_mask = __ballot(st == FL_INC);
+    if (inc_mask) {
+      const int k = __ffsll((long long)inc_mask) - 1;  // first (closest) inclusive flag
+      const int contrib = (l <= k) ? (int)val : 0;
+      prefix += wave_sum(contrib);
+      return prefix;
+    }
+    prefix += wave_sum((int)val);
+    j -= PVD_WAVE;
+  }
+}
+
+// ----------------------------------------------------------------------------- discrete
+// MODE: INJ = injected draws; IS = importance sampling (analytic trial); EXT = energies
+// from V_ext (walkers already moved), runtime component count.
+template <typename R, int NP, bool INJ, bool IS, bool EXT>
+__global__ __launch_bounds__(TILE) void k_discrete(StepArgs<R> A) {
+  __shared__ int s_tile, s_prefix, s_total, s_status;
+  __shared__ int s_sc[TILE / PVD_WAVE];
+  __shared__ double s_d[TILE / PVD_WAVE];
+  __shared__ int s_i[TILE / PVD_WAVE];
+  const int par = A.par;
+  Ctrl* C = A.ctrl;
+  if (threadIdx.x == 0) s_tile = (int)fetch_add_u(&C->ticket[par], 1u);
+  __syncthreads();
+  const int tile = s_tile;
+  const int n = C->n[par];
+  const int ntiles = (n + TILE - 1) / TILE;
+  if (C->status != ST_OK || tile >= ntiles) return;
+  const unsigned long long step = C->step[par];
+  const unsigned epoch = C->epoch[par];
+  const double vref = C->vref[par];
+  double dte = A.dt;
+  if (IS) dte = A.dt * ((double)C->acc[par] / (double)n);
+
+  const long long i = (long long)tile * TILE + threadIdx.x;
+  const bool valid = i < n;
+  R x[EXT ? 1 : NP];
+  double E = 0.0;
+  if (valid) {
+    if constexpr (EXT) {
+      E = A.V_ext[i];
+      if (IS) E = E + A.ke[i];
+    } else {
+      load_walker<R, NP>(A, i, x);
+      R ke = R(0);
+      if constexpr (IS) {
+        bool acc;
+        ke = is_move<R, NP, INJ>(A, i, step, x, acc);
+      } else {
+        R disp[NP];
+        draw_disp<R, NP, INJ>(A, i, step, disp);
+#pragma unroll
+        for (int p = 0; p < NP; ++p) x[p] = x[p] + disp[p];
+      }
+      const R v = pes_eval<R, NP>(A.pes, x);
+      E = IS ? (double)(v + ke) : (double)v;
+    }
+  }
+  // branching weight and copy count (pyvibdmc.py:392-407)
+  int c = valid ? 1 : 0;
+  int bad = 0;
+  if (valid && A.branch) {
+    const double u = draw_u<R, INJ>(A, i, step, RNG_BRANCH);
+    const double w = exp((-1. * (E - vref)) * dte);
+    if (!isfinite(w) || w > A.pop_hi + 1.0) {
+      bad = 1;
+      c = 0;
+    } else {
+      const double fl = floor(w);
+      c = (int)fl + (u < (w - fl) ? 1 : 0);
+    }
+  }
+  int total;
+  const int excl = block_excl_scan<TILE>(c, s_sc, total);
+  const double sce = block_sum<TILE>((double)c * E, s_d);
+  const int births = block_sum<TILE>(c > 1 ? c - 1 : 0, s_i);
+  const int deaths = block_sum<TILE>((valid && c == 0) ? 1 : 0, s_i);
+  const int badb = block_sum<TILE>(bad, s_i);
+
+  // publish the tile aggregate, then look back for the exclusive prefix
+  if (threadIdx.x == 0) {
+    TileRec r;
+    r.s0 = sce;
+    r.s1 = 0.0;
+    r.mx = 0.0;
+    r.count = total;
+    r.births = births;
+    r.deaths = deaths;
+    r.bad = badb;
+    A.rec[tile] = r;
+    st_release(&A.flags[tile], pack_flag(epoch, tile == 0 ? FL_INC : FL_AGG, (unsigned)total));
+  }
+  if (threadIdx.x < PVD_WAVE) {
+    int prefix = 0;
+    bool spun = false;
+    if (tile > 0) prefix = look_back(A.flags, tile, epoch, C, spun);
+    if (threadIdx.x == 0) {
+      if (spun) {
+        st_release_i(&C->status, ST_SPIN);
+      } else if (tile > 0) {
+        st_release(&A.flags[tile], pack_flag(epoch, FL_INC, (unsigned)(prefix + total)));
+      }
+      s_prefix = prefix;
+      s_status = spun ? 1 : 0;
+    }
+  }
+  __syncthreads();
+  if (s_status) return;
+  const long long base = (long long)s_prefix + excl;
+
+  // survivors to their np.repeat positions
+  if (valid && c > 0) {
+    for (int k = 0; k < c; ++k) {
+      const long long dst = base + k;
+      if (dst >= A.cap) break;
+      if constexpr (EXT) {
+        for (int p = 0; p < A.np; ++p) A.cout[p * A.cap + dst] = A.cin[p * A.cap + i];
+      } else {
+#pragma unroll
+        for (int p = 0; p < NP; ++p) A.cout[p * A.cap + dst] = x[p];
+      }
+      if (A.who_out) A.who_out[dst] = A.who_in[i];
+    }
+  }
+
+  // the last tile in scan order finalises the step
+  if (tile != ntiles - 1) return;
+  fence_acquire();
+  double s0 = 0.0;
+  int cnt = 0, nb = 0, nd = 0, nbad = 0;
+  for (int t = threadIdx.x; t < ntiles; t += TILE) {
+    const TileRec r = A.rec[t];
+    s0 += r.s0;
+    cnt += r.count;
+    nb += r.births;
+    nd += r.deaths;
+    nbad += r.bad;
+  }
+  s0 = block_sum<TILE>(s0, s_d);
+  cnt = block_sum<TILE>(cnt, s_i);
+  nb = block_sum<TILE>(nb, s_i);
+  nd = block_sum<TILE>(nd, s_i);
+  nbad = block_sum<TILE>(nbad, s_i);
+  if (threadIdx.x == 0) {
+    const int np1 = par ^ 1;
+    int st = ST_OK;
+    if (nbad) st = ST_WEIGHT;
+    else if ((double)cnt < A.pop_lo || (double)cnt > A.pop_hi) st = ST_POP;
+    else if (cnt > A.cap) st = ST_CAPACITY;
+    if (st != ST_OK) {
+      C->status = st;
+      C->status_step = (long long)step;
+    } else {
+      // calc_vref, discrete (pyvibdmc.py:655-661)
+      const double vbar = s0 / (double)cnt;
+      const double corr = (double)(cnt - (int)A.n0) / A.n0;
+      const double vnew = vbar - (A.alpha * corr);
+      C->n[np1] = cnt;
+      C->vref[np1] = vnew;
+      C->step[np1] = step + 1;
+      C->epoch[np1] = epoch + 1;
+      C->ticket[np1] = 0u;
+      if ((long long)step < A.tmax) {
+        A.tr_vref[step] = vnew;
+        A.tr_pop[step] = (double)cnt;
+        A.tr_births[step] = nb;
+        A.tr_deaths[step] = nd;
+        if (IS) {
+          C->eff_t = (step == 0 ? 0.0 : C->eff_t) + dte;
+          A.tr_dteff[step] = C->eff_t;
+        }
+      }
+    }
+  }
+}
+
+// -------------------------------------------------------------- acceptance pre-pass (IS)
+// Counts the accepted proposals of the coming step; the step kernel recomputes the same
+// proposals (same draws, same arithmetic) and uses the count for dt_eff.
+template <typename R, int NP, bool INJ>
+__global__ __launch_bounds__(TILE) void k_accept(StepArgs<R> A) {
+  __shared__ int s_i[TILE / PVD_WAVE];
+  __shared__ unsigned s_last;
+  const int par = A.par;
+  Ctrl* C = A.ctrl;
+  const int n = C->n[par];
+  const int ntiles = (n + TILE - 1) / TILE;
+  const int tile = blockIdx.x;
+  if (C->status != ST_OK || tile >= ntiles) return;
+  const unsigned long long step = C->step[par];
+  const long long i = (long long)tile * TILE + threadIdx.x;
+  int a = 0;
+  if (i < n) {
+    R x[NP];
+    load_walker<R, NP>(A, i, x);
+    bool acc;
+    is_move<R, NP, INJ>(A, i, step, x, acc);
+    a = acc ? 1 : 0;
+  }
+  a = block_sum<TILE>(a, s_i);
+  if (threadIdx.x == 0) {
+    A.rec[tile].count = a;
+    fence_release();
+    s_last = fetch_add_u(&C->done[par], 1u);
+  }
+  __syncthreads();
+  if (s_last != (unsigned)(ntiles - 1)) return;
+  fence_acquire();
+  int tot = 0;
+  for (int t = threadIdx.x; t < ntiles; t += TILE) tot += A.rec[t].count;
+  tot = block_sum<TILE>(tot, s_i);
+  if (threadIdx.x == 0) {
+    C->acc[par] = tot;
+    C->done[par] = 0u;
+  }
+}
+
+// ------------------------------------------------------------------ split path: the move
+// Moves walkers in place (plain or importance-sampled) so that an outside PES can be
+// evaluated on the moved walkers.  IS writes the local kinetic energy per walker and
+// counts acceptances.
+template <typename R, int NP, bool INJ, bool IS>
+__global__ __launch_bounds__(TILE) void k_move(StepArgs<R> A) {
+  __shared__ int s_i[TILE / PVD_WAVE];
+  __shared__ unsigned s_last;
+  const int par = A.par;
+  Ctrl* C = A.ctrl;
+  const int n = C->n[par];
+  const int ntiles = (n + TILE - 1) / TILE;
+  const int tile = blockIdx.x;
+  if (C->status != ST_OK || tile >= ntiles) return;
+  const unsigned long long step = C->step[par];
+  const long long i = (long long)tile * TILE + threadIdx.x;
+  int a = 0;
+  if (i < n) {
+    if constexpr (IS) {
+      R x[NP];
+      load_walker<R, NP>(A, i, x);
+      bool acc;
+      const R ke = is_move<R, NP, INJ>(A, i, step, x, acc);
+      a = acc ? 1 : 0;
+      A.ke[i] = (double)ke;
+#pragma unroll
+      for (int p = 0; p < NP; ++p) A.cout[p * A.cap + i] = x[p];
+    } else {
+      // runtime component count; draws in blocks of four components
+      for (int p0 = 0; p0 < A.np; p0 += 4) {
+        R z[4];
+        if constexpr (INJ) {
+          for (int q = 0; q < 4 && p0 + q < A.np; ++q) z[q] = (R)A.inj_disp[i * A.np + p0 + q];
+        } else {
+          const Philox4 r = A.rng.block((uint32_t)i, step, RNG_DISP, (uint32_t)(p0 / 4) | 0x80u);
+          if constexpr (sizeof(R) == 4) {
+            box_muller_f(r.v[0], r.v[1], z[0], z[1]);
+            box_muller_f(r.v[2], r.v[3], z[2], z[3]);
+          } else {
+            double a0, a1;
+            box_muller_d(r.v[0], r.v[1], r.v[2], r.v[3], a0, a1);
+            z[0] = a0;
+            z[1] = a1;
+            const Philox4 r2 = A.rng.block((uint32_t)i, step, RNG_DISP, (uint32_t)(p0 / 4) | 0xC0u);
+            box_muller_d(r2.v[0], r2.v[1], r2.v[2], r2.v[3], a0, a1);
+            z[2] = a0;
+            z[3] = a1;
+          }
+          for (int q = 0; q < 4; ++q) z[q] = (R)A.sigma[(p0 + q) < MAXNP ? p0 + q : 0] * z[q];
+        }
+        for (int q = 0; q < 4 && p0 + q < A.np; ++q) {
+          const long long o = (long long)(p0 + q) * A.cap + i;
+          A.cout[o] = A.cin[o] + z[q];
+        }
+      }
+    }
+  }
+  if constexpr (IS) {
+    a = block_sum<TILE>(a, s_i);
+    if (threadIdx.x == 0) {
+      A.rec[tile].count = a;
+      fence_release();
+      s_last = fetch_add_u(&C->done[par], 1u);
+    }
+    __syncthreads();
+    if (s_last != (unsigned)(ntiles - 1)) return;
+    fence_acquire();
+    int tot = 0;
+    for (int t = threadIdx.x; t < ntiles; t += TILE) tot += A.rec[t].count;
+    tot = block_sum<TILE>(tot, s_i);
+    if (threadIdx.x == 0) {
+      C->acc[par] = tot;
+      C->done[par] = 0u;
+    }
+  }
+}
+
+// ------------------------------------------------------------------------- continuous
+// Weights update in place (pyvibdmc.py:433, 833-837); walkers below the lower threshold
+// are flagged in per-wave masks for the split replay (pyvibdmc.py:436-437, 340-356).
+template <typename R, int NP, bool INJ, bool IS, bool EXT>
+__global__ __launch_bounds__(TILE) void k_cont(StepArgs<R> A) {
+  __shared__ double s_d[TILE / PVD_WAVE];
+  __shared__ int s_i[TILE / PVD_WAVE];
+  __shared__ unsigned s_last;
+  const int par = A.par;
+  Ctrl* C = A.ctrl;
+  const int n = C->n[par];
+  const int ntiles = (n + TILE - 1) / TILE;
+  const int tile = blockIdx.x;
+  if (C->status != ST_OK || tile >= ntiles) return;
+  const unsigned long long step = C->step[par];
+  const double vref = C->vref[par];
+  double dte = A.dt;
+  if (IS) dte = A.dt * ((double)C->acc[par] / (double)n);
+  const long long i = (long long)tile * TILE + threadIdx.x;
+  const bool valid = i < n;
+  double E = 0.0, w = 0.0;
+  bool kill = false;
+  if (valid) {
+    if constexpr (EXT) {
+      E = A.V_ext[i];
+      if (IS) E = E + A.ke[i];
+    } else {
+      R x[NP];
+      load_walker<R, NP>(A, i, x);
+      R ke = R(0);
+      if constexpr (IS) {
+        bool acc;
+        ke = is_move<R, NP, INJ>(A, i, step, x, acc);
+      } else {
+        R disp[NP];
+        draw_disp<R, NP, INJ>(A, i, step, disp);
+#pragma unroll
+        for (int p = 0; p < NP; ++p) x[p] = x[p] + disp[p];
+      }
+#pragma unroll
+      for (int p = 0; p < NP; ++p) A.cout[p * A.cap + i] = x[p];
+      const R v = pes_eval<R, NP>(A.pes, x);
+      E = IS ? (double)(v + ke) : (double)v;
+    }
+    w = A.w[i] * exp((-1.0 * (E - vref)) * dte);
+    A.w[i] = w;
+    A.E[i] = E;
+    kill = A.branch && (w < A.thresh_lo);
+  }
+  const unsigned long long km = __ballot(kill);
+  if (lane_id() == 0) A.kmask[(long long)tile * (TILE / PVD_WAVE) + threadIdx.x / PVD_WAVE] = km;
+  const double swe = block_sum<TILE>(w * E, s_d);
+  const double sw = block_sum<TILE>(w, s_d);
+  const double mx = block_max<TILE>(valid ? w : 0.0, s_d);
+  const int nk = block_sum<TILE>(kill ? 1 : 0, s_i);
+  if (threadIdx.x == 0) {
+    TileRec r;
+    r.s0 = swe;
+    r.s1 = sw;
+    r.mx = mx;
+    r.count = nk;
+    r.births = 0;
+    r.deaths = 0;
+    r.bad = 0;
+    A.rec[tile] = r;
+    fence_release();
+    s_last = fetch_add_u(&C->done[par], 1u);
+  }
+  __syncthreads();
+  if (s_last != (unsigned)(ntiles - 1)) return;
+  fence_acquire();
+  double a0 = 0.0, a1 = 0.0;
+  int k = 0;
+  for (int t = threadIdx.x; t < ntiles; t += TILE) {
+    a0 += A.rec[t].s0;
+    a1 += A.rec[t].s1;
+    k += A.rec[t].count;
+  }
+  a0 = block_sum<TILE>(a0, s_d);
+  a1 = block_sum<TILE>(a1, s_d);
+  k = block_sum<TILE>(k, s_i);
+  if (threadIdx.x == 0) {
+    const int np1 = par ^ 1;
+    C->done[par] = 0u;
+    C->kills[par] = k;
+    C->sumw[par] = a1;
+    C->sumwe[par] = a0;
+    C->n[np1] = n;
+    C->step[np1] = step + 1;
+    C->epoch[np1] = C->epoch[par] + 1;
+    if (IS && (long long)step < A.tmax) {
+      C->eff_t = (step == 0 ? 0.0 : C->eff_t) + dte;
+      A.tr_dteff[step] = C->eff_t;
+    }
+    if (k == 0) {
+      // calc_vref, continuous (pyvibdmc.py:659-661)
+      const double vnew = a0 / a1 - (A.alpha * ((a1 - A.n0) / A.n0));
+      C->vref[np1] = vnew;
+      if ((long long)step < A.tmax) {
+        A.tr_vref[step] = vnew;
+        A.tr_pop[step] = a1;
+        A.tr_births[step] = 0;
+      }
+    }
+  }
+}
+
+// Exact replay of the sequential split loop (pyvibdmc.py:344-356) for all walkers below
+// the threshold, in one workgroup.
+//
+// The loop repeatedly takes the walker of largest weight (first index on ties), halves
+// it and gives the other half to the next walker on the kill list.  The chosen weights
+// never increase, and halving moves a weight from [L, H) to below H/2 <= L.  So for any
+// interval [L, H) with H <= 2L that lies above everything still to be chosen, the
+// entries inside it are chosen in order of (weight desc, index asc) before any piece
+// they create.  The replay walks such intervals downwards, sorting each one in LDS.
+//
+// Above the threshold no killed walker can be a candidate, so an interval's kills are
+// assigned to its sorted members in parallel.  Once the intervals reach below the
+// threshold, killed walkers that have not had their turn yet are candidates too, and a
+// piece that sits in a killed slot is overwritten when that slot's turn comes; there the
+// interval is replayed entry by entry by one thread with that validity rule.  All
+// copies read their sources before any write, and the corrections to sum(w) and
+// sum(w*E) use the pre-split values.
+constexpr int RP_NT = 1024;
+constexpr int RP_CAP = 2048;   // interval members held in LDS
+constexpr int RP_KCAP = 4096;  // kills per step
+constexpr int RP_PCAP = RP_KCAP * 2;  // pending pieces per buffer
+constexpr int RP_FCAP = RP_KCAP * 4;  // final pieces
+
+PVD_DEV bool piece_before(const Piece& a, const Piece& b) {
+  return a.v > b.v || (a.v == b.v && a.slot < b.slot);
+}
+
+// kill index of `slot` in this launch (tagged with the launch epoch), or INT_MAX
+PVD_DEV int kill_pos(const unsigned long long* tag, int slot, unsigned long long epoch) {
+  const unsigned long long t = tag[slot];
+  return (t >> 32) == epoch ? (int)(t & 0xFFFFFFFFull) : 0x7FFFFFFF;
+}
+
+// a piece is live at kill index q unless its slot's own turn (earlier than q) overwrote it
+PVD_DEV bool piece_live(const Piece& e, int q, const unsigned long long* tag, unsigned long long epoch) {
+  const int kp = kill_pos(tag, e.slot, epoch);
+  return !(kp < q && e.birth < kp);
+}
+
+template <typename R>
+__global__ __launch_bounds__(RP_NT) void k_cont_replay(StepArgs<R> A) {
+  __shared__ Piece cur[RP_CAP];
+  __shared__ int s_cnt, s_ncur, s_npend, s_nfin, s_fail, s_next;
+  __shared__ double s_d[RP_NT / PVD_WAVE];
+  __shared__ int s_i[RP_NT / PVD_WAVE];
+  const int par = A.par;
+  Ctrl* C = A.ctrl;
+  const int np1 = par ^ 1;
+  const int K = C->kills[par];
+  if (C->status != ST_OK || K == 0) return;
+  const int n = C->n[par];
+  const int ntiles = (n + TILE - 1) / TILE;
+  const unsigned long long step = C->step[par];
+  const unsigned long long ep = (unsigned long long)C->epoch[par];
+  const unsigned long long tagbase = ep << 32;
+  const int tid = threadIdx.x;
+  int* kl = A.klist;
+  if (K > RP_KCAP) {
+    if (tid == 0) {
+      C->status = ST_BRANCH;
+      C->status_step = (long long)step;
+    }
+    return;
+  }
+  // 1. kill list in ascending slot order from the per-wave masks, and each killed
+  //    slot's position in it
+  const int nwords = ntiles * (TILE / PVD_WAVE);
+  if (tid == 0) s_cnt = 0;
+  __syncthreads();
+  for (int base = 0; base < nwords; base += RP_NT) {
+    const int wi = base + tid;
+    const unsigned long long m = wi < nwords ? A.kmask[wi] : 0ull;
+    const int pc = __popcll(m);
+    int tot;
+    const int ex = block_excl_scan<RP_NT>(pc, s_i, tot);
+    unsigned long long mm = m;
+    int o = s_cnt + ex;
+    while (mm) {
+      const int b = __ffsll((long long)mm) - 1;
+      mm &= mm - 1ull;
+      const int slot = wi * PVD_WAVE + b;
+      kl[o] = slot;
+      A.killtag[slot] = tagbase | (unsigned long long)o;
+      ++o;
+    }
+    __syncthreads();
+    if (tid == 0) s_cnt += tot;
+    __syncthreads();
+  }
+  // 2. global maximum weight; first interval [gm, 2 gm)
+  double gm = 0.0;
+  for (int t = tid; t < ntiles; t += RP_NT) gm = fmax(gm, A.rec[t].mx);
+  gm = block_max<RP_NT>(gm, s_d);
+  if (tid == 0) {
+    s_npend = 0;
+    s_nfin = 0;
+    s_fail = 0;
+  }
+  __syncthreads();
+  double H = 2.0 * gm;
+  int next = 0;
+  int pbuf = 0;  // which half of `pend` holds the live pending list
+  int guard = 0;
+  const int wv = tid / PVD_WAVE, l = lane_id();
+  while (next < K) {
+    // choose L in [H/2, H) so that the interval fits in LDS
+    double L = 0.5 * H;
+    for (int attempt = 0;; ++attempt) {
+      int c = 0;
+      for (int t = wv; t < ntiles; t += RP_NT / PVD_WAVE) {
+        if (A.rec[t].mx < L) continue;
+        const long long s0 = (long long)t * TILE + l;
+        for (int q = 0; q < TILE / PVD_WAVE; ++q) {
+          const long long ss = s0 + q * PVD_WAVE;
+          if (ss < n) {
+            const double w = A.w[ss];
+            c += (w >= L && w < H) ? 1 : 0;
+          }
+        }
+      }
+      c = block_sum<RP_NT>(c, s_i);
+      int pin = 0;
+      for (int q = tid; q < s_npend; q += RP_NT) {
+        const double v = A.pend[pbuf * RP_PCAP + q].v;
+        pin += (v >= L && v < H) ? 1 : 0;
+      }
+      pin = block_sum<RP_NT>(pin, s_i);
+      if (c + pin <= RP_CAP) break;
+      L = 0.5 * (L + H);
+      if (attempt > 60) {
+        if (tid == 0) s_fail = 1;
+        break;
+      }
+    }
+    __syncthreads();
+    if (s_fail || !(L > 1e-300) || !(L < H) || ++guard > 4096) {
+      if (tid == 0) {
+        C->status = ST_BRANCH;
+        C->status_step = (long long)step;
+      }
+      return;
+    }
+    const bool degenerate = L < A.thresh_lo;
+    // gather the interval: in-range pending pieces, then original walkers in range
+    if (tid == 0) s_ncur = 0;
+    const int npend = s_npend;
+    const Piece* pin_list = A.pend + pbuf * RP_PCAP;
+    Piece* pout_list = A.pend + (pbuf ^ 1) * RP_PCAP;
+    __syncthreads();
+    if (tid == 0) s_npend = 0;
+    __syncthreads();
+    for (int q = tid; q < npend; q += RP_NT) {
+      const Piece pc = pin_list[q];
+      if (pc.v >= L && pc.v < H) cur[atomicAdd(&s_ncur, 1)] = pc;
+      else pout_list[atomicAdd(&s_npend, 1)] = pc;
+    }
+    for (int t = wv; t < ntiles; t += RP_NT / PVD_WAVE) {
+      if (A.rec[t].mx < L) continue;
+      const long long s0 = (long long)t * TILE + l;
+      for (int q = 0; q < TILE / PVD_WAVE; ++q) {
+        const long long ss = s0 + q * PVD_WAVE;
+        if (ss < n) {
+          const double w = A.w[ss];
+          if (w >= L && w < H) {
+            Piece pc;
+            pc.v = w;
+            pc.slot = (int)ss;
+            pc.src = (int)ss;
+            pc.birth = -1;
+            pc.pad = 0;
+            cur[atomicAdd(&s_ncur, 1)] = pc;
+          }
+        }
+      }
+    }
+    __syncthreads();
+    const int ncur = s_ncur;
+    // bitonic sort of cur[0, P), P = next power of two, sentinels at the end
+    int P = 1;
+    while (P < ncur) P <<= 1;
+    for (int q = ncur + tid; q < P; q += RP_NT) {
+      Piece z;
+      z.v = -1.0;
+      z.slot = 0x7FFFFFFF;
+      z.src = -1;
+      z.birth = -1;
+      z.pad = 0;
+      cur[q] = z;
+    }
+    __syncthreads();
+    for (int kk = 2; kk <= P; kk <<= 1) {
+      for (int jj = kk >> 1; jj > 0; jj >>= 1) {
+        for (int q = tid; q < P; q += RP_NT) {
+          const int ixj = q ^ jj;
+          if (ixj > q) {
+            const bool up = (q & kk) == 0;
+            const Piece a = cur[q], b = cur[ixj];
+            if (up ? piece_before(b, a) : piece_before(a, b)) {
+              cur[q] = b;
+              cur[ixj] = a;
+            }
+          }
+        }
+        __syncthreads();
+      }
+    }
+    if (!degenerate) {
+      // members [0, m) are chosen in order and split; the rest are final
+      const int m = min(ncur, K - next);
+      for (int q = tid; q < ncur; q += RP_NT) {
+        const Piece pc = cur[q];
+        if (q < m) {
+          Piece a = pc, b = pc;
+          a.v = pc.v * 0.5;
+          a.birth = next + q;
+          b.v = a.v;
+          b.birth = next + q;
+          b.slot = kl[next + q];
+          const int o = atomicAdd(&s_npend, 2);
+          if (o + 2 <= RP_PCAP) {
+            pout_list[o] = a;
+            pout_list[o + 1] = b;
+          } else {
+            s_fail = 1;
+          }
+        } else if (pc.birth >= 0) {
+          const int o = atomicAdd(&s_nfin, 1);
+          if (o < RP_FCAP) A.fin[o] = pc;
+          else s_fail = 1;
+        }
+      }
+      next += m;
+    } else {
+      // entry-by-entry replay of this interval by one thread
+      if (tid == 0) {
+        int ptr = 0, q = next;
+        for (; q < K; ++q) {
+          while (ptr < ncur && !piece_live(cur[ptr], q, A.killtag, ep)) ++ptr;
+          if (ptr >= ncur) break;
+          const Piece e = cur[ptr++];
+          const int ks = kl[q];
+          Piece a = e;
+          a.v = e.v * 0.5;
+          a.birth = q;
+          if (e.slot == ks) {
+            if (s_npend + 1 <= RP_PCAP) pout_list[s_npend++] = a;
+            else s_fail = 1;
+          } else {
+            Piece b = a;
+            b.slot = ks;
+            if (s_npend + 2 <= RP_PCAP) {
+              pout_list[s_npend++] = a;
+              pout_list[s_npend++] = b;
+            } else {
+              s_fail = 1;
+            }
+          }
+        }
+        for (; ptr < ncur; ++ptr) {
+          if (cur[ptr].birth >= 0) {
+            if (s_nfin < RP_FCAP) A.fin[s_nfin++] = cur[ptr];
+            else s_fail = 1;
+          }
+        }
+        s_next = q;
+      }
+      __syncthreads();
+      next = s_next;
+    }
+    __syncthreads();
+    if (s_fail) {
+      if (tid == 0) {
+        C->status = ST_BRANCH;
+        C->status_step = (long long)step;
+      }
+      return;
+    }
+    H = L;
+    pbuf ^= 1;
+  }
+  // every still-pending piece is final
+  {
+    const int npend = s_npend;
+    const Piece* pl = A.pend + pbuf * RP_PCAP;
+    for (int q = tid; q < npend; q += RP_NT) {
+      const int o = atomicAdd(&s_nfin, 1);
+      if (o < RP_FCAP) A.fin[o] = pl[q];
+      else s_fail = 1;
+    }
+    __syncthreads();
+    if (s_fail) {
+      if (tid == 0) {
+        C->status = ST_BRANCH;
+        C->status_step = (long long)step;
+      }
+      return;
+    }
+  }
+  // read phase: sources and corrections from the pre-split state (pieces overwritten by
+  // their slot's own turn are dropped)
+  const int nfin = s_nfin;
+  const int np = A.np;
+  double lsw = 0.0, lswe = 0.0;
+  for (int q = tid; q < nfin; q += RP_NT) {
+    Piece pc = A.fin[q];
+    if (!piece_live(pc, K, A.killtag, ep)) {
+      pc.slot = -1;
+      A.fin[q] = pc;
+      continue;
+    }
+    const int s = pc.slot;
+    const double wold = A.w[s];
+    const double esrc = A.E[pc.src];
+    lsw += pc.v - wold;
+    lswe += pc.v * esrc - wold * A.E[s];
+    if (s != pc.src) {
+      for (int p = 0; p < np; ++p) A.rstage[(long long)q * (np + 2) + p] = (double)A.cout[p * A.cap + pc.src];
+      A.rstage[(long long)q * (np + 2) + np] = esrc;
+      A.rstage[(long long)q * (np + 2) + np + 1] = A.who_out ? (double)A.who_out[pc.src] : 0.0;
+    }
+  }
+  lsw = block_sum<RP_NT>(lsw, s_d);
+  lswe = block_sum<RP_NT>(lswe, s_d);  // contains a barrier: all reads are done
+  for (int q = tid; q < nfin; q += RP_NT) {
+    const Piece pc = A.fin[q];
+    if (pc.slot < 0) continue;
+    A.w[pc.slot] = pc.v;
+    if (pc.slot != pc.src) {
+      for (int p = 0; p < np; ++p) A.cout[p * A.cap + pc.slot] = (R)A.rstage[(long long)q * (np + 2) + p];
+      A.E[pc.slot] = A.rstage[(long long)q * (np + 2) + np];
+      if (A.who_out) A.who_out[pc.slot] = (int)A.rstage[(long long)q * (np + 2) + np + 1];
+    }
+  }
+  if (tid == 0) {
+    const double a1 = C->sumw[par] + lsw;
+    const double a0 = C->sumwe[par] + lswe;
+    const double vnew = a0 / a1 - (A.alpha * ((a1 - A.n0) / A.n0));
+    C->vref[np1] = vnew;
+    if ((long long)step < A.tmax) {
+      A.tr_vref[step] = vnew;
+      A.tr_pop[step] = a1;
+      A.tr_births[step] = K;
+    }
+  }
+}
+
+}  // namespace pvd

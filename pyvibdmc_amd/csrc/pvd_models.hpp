@@ -1,0 +1,389 @@
+// Potential-energy surfaces and trial wave functions evaluated per walker in registers.
+//
+// A walker is NP = M*d Cartesian components x[a*d + c] (atom a, component c), the same
+// order as the reference's (N, M, d) arrays (pyvibdmc.py:156-169).  Each model is a
+// small struct passed to kernels by value; `Real` is float (production) or double
+// (parity / FP64 runs).  Operation order in the FP64 forms follows the NumPy
+// expressions of the oracle so that FP64 results agree with it to a few ulps.
+#pragma once
+#include "pvd_common.hpp"
+
+namespace pvd {
+
+enum PesKind : int {
+  PES_EXTERNAL = 0,    // energies supplied from outside (host callback or surrogate)
+  PES_HARMONIC_X = 1,  // V = c*(x*x), x = first coordinate of atom 0
+  PES_MORSE_1D = 2,    // V = De*(t*t), t = 1 - exp(-alpha*x)
+  PES_MORSE_BEND = 3,  // two Morse O-H stretches + harmonic bend, atoms (h1, h2, o)
+  PES_CLUSTER6 = 4,    // two Morse+bend monomers + O-O Morse + inter-monomer Coulomb
+};
+
+struct PesParams {
+  int kind;
+  int idx[4];   // atom indices (MORSE_BEND: h1, h2, o)
+  double p[12]; // HARMONIC_X: c | MORSE_1D: De, -alpha | MORSE_BEND: De, a, re, kb, theta0 |
+                // CLUSTER6: De, a, re, kb, theta0, De_oo, a_oo, re_oo, qO, qH
+};
+
+template <typename R>
+PVD_DEV R pvd_exp(R x);
+template <>
+PVD_DEV float pvd_exp<float>(float x) { return expf(x); }
+template <>
+PVD_DEV double pvd_exp<double>(double x) { return exp(x); }
+template <typename R>
+PVD_DEV R pvd_sqrt(R x) { return sqrt(x); }
+template <>
+PVD_DEV float pvd_sqrt<float>(float x) { return __fsqrt_rn(x); }
+template <typename R>
+PVD_DEV R pvd_acos(R x);
+template <>
+PVD_DEV float pvd_acos<float>(float x) { return acosf(x); }
+template <>
+PVD_DEV double pvd_acos<double>(double x) { return acos(x); }
+template <typename R>
+PVD_DEV R pvd_log(R x);
+template <>
+PVD_DEV float pvd_log<float>(float x) { return logf(x); }
+template <>
+PVD_DEV double pvd_log<double>(double x) { return log(x); }
+
+template <typename R>
+PVD_DEV R clamp1(R c) {
+  return c > R(1) ? R(1) : (c < R(-1) ? R(-1) : c);
+}
+
+template <typename R>
+PVD_DEV R norm3(R a, R b, R c) {
+  return pvd_sqrt<R>(a * a + b * b + c * c);
+}
+
+// Morse + bend of one triatomic whose atoms start at component offsets h1, h2, o (d = 3).
+template <typename R>
+PVD_DEV R morse_bend_v(const R* x, int h1, int h2, int o, R De, R a, R re, R hk, R th0) {
+  const R u0 = x[h1 + 0] - x[o + 0], u1 = x[h1 + 1] - x[o + 1], u2 = x[h1 + 2] - x[o + 2];
+  const R v0 = x[h2 + 0] - x[o + 0], v1 = x[h2 + 1] - x[o + 1], v2 = x[h2 + 2] - x[o + 2];
+  const R r1 = norm3(u0, u1, u2);
+  const R r2 = norm3(v0, v1, v2);
+  const R dot = u0 * v0 + u1 * v1 + u2 * v2;
+  const R c = clamp1(dot / (r1 * r2));
+  const R th = pvd_acos<R>(c);
+  const R m1 = R(1) - pvd_exp<R>(-a * (r1 - re));
+  const R m2 = R(1) - pvd_exp<R>(-a * (r2 - re));
+  const R dt = th - th0;
+  return De * (m1 * m1) + De * (m2 * m2) + hk * (dt * dt);
+}
+
+// gradient of morse_bend_v accumulated into g (NP components)
+template <typename R>
+PVD_DEV void morse_bend_g(const R* x, R* g, int h1, int h2, int o, R De, R a, R re, R kb, R th0) {
+  R u[3], v[3];
+#pragma unroll
+  for (int c = 0; c < 3; ++c) {
+    u[c] = x[h1 + c] - x[o + c];
+    v[c] = x[h2 + c] - x[o + c];
+  }
+  const R r1 = norm3(u[0], u[1], u[2]);
+  const R r2 = norm3(v[0], v[1], v[2]);
+  const R e1 = pvd_exp<R>(-a * (r1 - re));
+  const R e2 = pvd_exp<R>(-a * (r2 - re));
+  const R g1 = R(2) * De * (R(1) - e1) * a * e1;
+  const R g2 = R(2) * De * (R(1) - e2) * a * e2;
+  R uh[3], vh[3];
+#pragma unroll
+  for (int c = 0; c < 3; ++c) {
+    uh[c] = u[c] / r1;
+    vh[c] = v[c] / r2;
+  }
+  const R cth = clamp1(uh[0] * vh[0] + uh[1] * vh[1] + uh[2] * vh[2]);
+  const R s = pvd_sqrt<R>(R(1) - cth * cth);
+  const R gt = kb * (pvd_acos<R>(cth) - th0);
+#pragma unroll
+  for (int c = 0; c < 3; ++c) {
+    const R dcu = (vh[c] - cth * uh[c]) / r1;
+    const R dcv = (uh[c] - cth * vh[c]) / r2;
+    const R dtu = -dcu / s, dtv = -dcv / s;
+    g[h1 + c] += g1 * uh[c] + gt * dtu;
+    g[h2 + c] += g2 * vh[c] + gt * dtv;
+    g[o + c] += -g1 * uh[c] - g2 * vh[c] - gt * (dtu + dtv);
+  }
+}
+
+// x components of atoms idx[0], idx[1], idx[2] into g[0..9) without dynamic register indexing
+template <typename R, int NP>
+PVD_DEV void gather3(const R* x, const int* idx, R* g) {
+#pragma unroll
+  for (int a = 0; a < 3; ++a) {
+    const int b = 3 * idx[a];
+#pragma unroll
+    for (int c = 0; c < 3; ++c) {
+      R v = R(0);
+#pragma unroll
+      for (int p = 0; p < NP; ++p)
+        if (p == b + c) v = x[p];
+      g[3 * a + c] = v;
+    }
+  }
+}
+
+// Evaluate a device PES.  NP is the number of Cartesian components per walker.
+template <typename R, int NP>
+PVD_DEV R pes_eval(const PesParams& P, const R* x) {
+  switch (P.kind) {
+    case PES_HARMONIC_X: {
+      const R x0 = x[0];
+      return R(P.p[0]) * (x0 * x0);
+    }
+    case PES_MORSE_1D: {
+      const R t = R(1) - pvd_exp<R>(R(P.p[1]) * x[0]);
+      return R(P.p[0]) * (t * t);
+    }
+    case PES_MORSE_BEND:
+      if constexpr (NP >= 9) {
+        // gather (h1, h2, o) with compile-time selects so the walker stays in registers
+        R g[9];
+        gather3<R, NP>(x, P.idx, g);
+        return morse_bend_v<R>(g, 0, 3, 6, R(P.p[0]), R(P.p[1]), R(P.p[2]), R(0.5 * P.p[3]), R(P.p[4]));
+      }
+      break;
+    case PES_CLUSTER6:
+      if constexpr (NP == 18) {
+        const R De = R(P.p[0]), a = R(P.p[1]), re = R(P.p[2]), hk = R(0.5 * P.p[3]), th0 = R(P.p[4]);
+        R v = morse_bend_v<R>(x, 3, 6, 0, De, a, re, hk, th0) + morse_bend_v<R>(x, 12, 15, 9, De, a, re, hk, th0);
+        const R roo = norm3(x[0] - x[9], x[1] - x[10], x[2] - x[11]);
+        const R m = R(1) - pvd_exp<R>(-R(P.p[6]) * (roo - R(P.p[7])));
+        v = v + R(P.p[5]) * (m * m);
+        const R q[3] = {R(P.p[8]), R(P.p[9]), R(P.p[9])};
+#pragma unroll
+        for (int i = 0; i < 3; ++i)
+#pragma unroll
+          for (int j = 0; j < 3; ++j) {
+            const R r = norm3(x[3 * i] - x[9 + 3 * j], x[3 * i + 1] - x[10 + 3 * j], x[3 * i + 2] - x[11 + 3 * j]);
+            v = v + (q[i] * q[j]) / r;
+          }
+        return v;
+      }
+      break;
+    default:
+      break;
+  }
+  return R(0) / R(0);  // unsupported: NaN trips the weight guard
+}
+
+// dV/dx for the analytic surfaces (forces are -g).
+template <typename R, int NP>
+PVD_DEV void pes_grad(const PesParams& P, const R* x, R* g) {
+#pragma unroll
+  for (int p = 0; p < NP; ++p) g[p] = R(0);
+  switch (P.kind) {
+    case PES_HARMONIC_X:
+      g[0] = R(2) * R(P.p[0]) * x[0];
+      break;
+    case PES_MORSE_1D: {
+      const R e = pvd_exp<R>(R(P.p[1]) * x[0]);
+      g[0] = R(2) * R(P.p[0]) * (R(1) - e) * (-R(P.p[1])) * e;
+      break;
+    }
+    case PES_MORSE_BEND:
+      if constexpr (NP >= 9) {
+        R xg[9], gg[9];
+        gather3<R, NP>(x, P.idx, xg);
+#pragma unroll
+        for (int q = 0; q < 9; ++q) gg[q] = R(0);
+        morse_bend_g<R>(xg, gg, 0, 3, 6, R(P.p[0]), R(P.p[1]), R(P.p[2]), R(P.p[3]), R(P.p[4]));
+        // scatter back
+#pragma unroll
+        for (int a = 0; a < 3; ++a)
+#pragma unroll
+          for (int c = 0; c < 3; ++c)
+#pragma unroll
+            for (int p = 0; p < NP; ++p)
+              if (p == 3 * P.idx[a] + c) g[p] += gg[3 * a + c];
+      }
+      break;
+    case PES_CLUSTER6:
+      if constexpr (NP == 18) {
+        const R De = R(P.p[0]), a = R(P.p[1]), re = R(P.p[2]), kb = R(P.p[3]), th0 = R(P.p[4]);
+        morse_bend_g<R>(x, g, 3, 6, 0, De, a, re, kb, th0);
+        morse_bend_g<R>(x, g, 12, 15, 9, De, a, re, kb, th0);
+        {
+          R d[3];
+#pragma unroll
+          for (int c = 0; c < 3; ++c) d[c] = x[c] - x[9 + c];
+          const R r = norm3(d[0], d[1], d[2]);
+          const R e = pvd_exp<R>(-R(P.p[6]) * (r - R(P.p[7])));
+          const R s = R(2) * R(P.p[5]) * (R(1) - e) * R(P.p[6]) * e / r;
+#pragma unroll
+          for (int c = 0; c < 3; ++c) {
+            g[c] += s * d[c];
+            g[9 + c] -= s * d[c];
+          }
+        }
+        const R q[3] = {R(P.p[8]), R(P.p[9]), R(P.p[9])};
+#pragma unroll
+        for (int i = 0; i < 3; ++i)
+#pragma unroll
+          for (int j = 0; j < 3; ++j) {
+            R d[3];
+#pragma unroll
+            for (int c = 0; c < 3; ++c) d[c] = x[3 * i + c] - x[9 + 3 * j + c];
+            const R r = norm3(d[0], d[1], d[2]);
+            const R s = -(q[i] * q[j]) / (r * r * r);
+#pragma unroll
+            for (int c = 0; c < 3; ++c) {
+              g[3 * i + c] += s * d[c];
+              g[9 + 3 * j + c] -= s * d[c];
+            }
+          }
+      }
+      break;
+    default:
+      break;
+  }
+}
+
+// ------------------------------------------------------------------------------ trial psi
+enum TrialKind : int {
+  TRIAL_NONE = 0,
+  TRIAL_HARMONIC_1D = 1,  // psi = c exp(-alpha x^2/2) on coordinate 0
+  TRIAL_GAUSS = 2,        // product of Gaussians in bond lengths and bond angles
+};
+
+#define PVD_MAX_TB 8
+#define PVD_MAX_TA 4
+
+struct TrialParams {
+  int kind;
+  int nb, na;
+  int bi[PVD_MAX_TB], bj[PVD_MAX_TB];
+  int ai[PVD_MAX_TA], aj[PVD_MAX_TA], ak[PVD_MAX_TA];
+  double balpha[PVD_MAX_TB], br0[PVD_MAX_TB];
+  double aalpha[PVD_MAX_TA], ath0[PVD_MAX_TA];
+  double c0, c1, c2;  // HARMONIC_1D: (alpha/pi)^0.25, alpha, alpha^2 | GAUSS: log normalisation
+};
+
+// Result of a trial evaluation at one walker: psi, d ln psi / dx and d2 psi / dx2 / psi
+// (diagonal, per Cartesian component), i.e. exactly what a reference ``deriv_function``
+// returns (imp_samp.py:21-26).
+template <typename R, int NP>
+struct TrialOut {
+  R psi;
+  R lnpsi;
+  R d1[NP];
+  R d2[NP];
+};
+
+template <typename R, int NP>
+PVD_DEV void trial_eval(const TrialParams& T, const R* x, TrialOut<R, NP>& o) {
+  if (T.kind == TRIAL_HARMONIC_1D) {
+    // harm_trial_wfn.py:6-40: trl = c*e, derv = c*(-alpha x)*e, sderv = c*(alpha^2 x^2 - alpha)*e
+    const R c = R(T.c0), al = R(T.c1), al2 = R(T.c2);
+    const R x0 = x[0];
+    const R e = pvd_exp<R>(((-al) * (x0 * x0)) / R(2));
+    const R trl = c * e;
+    const R derv = (c * ((-al) * x0)) * e;
+    const R sderv = (c * (al2 * (x0 * x0) - al)) * e;
+    o.psi = trl;
+    o.lnpsi = pvd_log<R>(c) + ((-al) * (x0 * x0)) / R(2);
+#pragma unroll
+    for (int p = 0; p < NP; ++p) {
+      o.d1[p] = R(0);
+      o.d2[p] = R(0);
+    }
+    o.d1[0] = derv / trl;
+    o.d2[0] = sderv / trl;
+    return;
+  }
+  // TRIAL_GAUSS: ln psi = lnnorm - sum_k alpha_k (q_k - q0_k)^2 / 2
+  R g[NP], h[NP];
+#pragma unroll
+  for (int p = 0; p < NP; ++p) {
+    g[p] = R(0);
+    h[p] = R(0);
+  }
+  R q2 = R(0);
+  if constexpr (NP % 3 == 0) {
+    for (int k = 0; k < T.nb; ++k) {
+      const int i = 3 * T.bi[k], j = 3 * T.bj[k];
+      const R al = R(T.balpha[k]);
+      R u[3];
+#pragma unroll
+      for (int c = 0; c < 3; ++c) u[c] = x[i + c] - x[j + c];
+      const R r = norm3(u[0], u[1], u[2]);
+      const R dq = r - R(T.br0[k]);
+      q2 += al * (dq * dq);
+#pragma unroll
+      for (int c = 0; c < 3; ++c) {
+        const R e = u[c] / r;
+        const R d2 = (R(1) - e * e) / r;
+        const R t1 = -al * dq * e;
+        const R t2 = -al * (e * e + dq * d2);
+#pragma unroll
+        for (int p = 0; p < NP; ++p) {
+          if (p == i + c) {
+            g[p] += t1;
+            h[p] += t2;
+          }
+          if (p == j + c) {
+            g[p] -= t1;
+            h[p] += t2;
+          }
+        }
+      }
+    }
+    for (int k = 0; k < T.na; ++k) {
+      const int i = 3 * T.ai[k], j = 3 * T.aj[k], kk = 3 * T.ak[k];
+      const R al = R(T.aalpha[k]);
+      R u[3], v[3];
+#pragma unroll
+      for (int c = 0; c < 3; ++c) {
+        u[c] = x[i + c] - x[j + c];
+        v[c] = x[kk + c] - x[j + c];
+      }
+      const R ru = norm3(u[0], u[1], u[2]), rv = norm3(v[0], v[1], v[2]);
+      R uh[3], vh[3];
+#pragma unroll
+      for (int c = 0; c < 3; ++c) {
+        uh[c] = u[c] / ru;
+        vh[c] = v[c] / rv;
+      }
+      const R cth = clamp1(uh[0] * vh[0] + uh[1] * vh[1] + uh[2] * vh[2]);
+      const R s = pvd_sqrt<R>(R(1) - cth * cth);
+      const R dq = pvd_acos<R>(cth) - R(T.ath0[k]);
+      q2 += al * (dq * dq);
+#pragma unroll
+      for (int c = 0; c < 3; ++c) {
+        const R dcu = (vh[c] - cth * uh[c]) / ru;
+        const R dcv = (uh[c] - cth * vh[c]) / rv;
+        const R duu = (R(-2) * uh[c] * vh[c] - cth + R(3) * cth * uh[c] * uh[c]) / (ru * ru);
+        const R dvv = (R(-2) * uh[c] * vh[c] - cth + R(3) * cth * vh[c] * vh[c]) / (rv * rv);
+        const R duv = (R(1) - uh[c] * uh[c] - vh[c] * vh[c] + cth * uh[c] * vh[c]) / (ru * rv);
+        const R dcs[3] = {dcu, dcv, -(dcu + dcv)};
+        const R d2cs[3] = {duu, dvv, duu + R(2) * duv + dvv};
+        const int at[3] = {i + c, kk + c, j + c};
+#pragma unroll
+        for (int z = 0; z < 3; ++z) {
+          const R dth = -dcs[z] / s;
+          const R d2th = -d2cs[z] / s - cth * dcs[z] * dcs[z] / (s * s * s);
+          const R t1 = -al * dq * dth;
+          const R t2 = -al * (dth * dth + dq * d2th);
+#pragma unroll
+          for (int p = 0; p < NP; ++p)
+            if (p == at[z]) {
+              g[p] += t1;
+              h[p] += t2;
+            }
+        }
+      }
+    }
+  }
+  o.lnpsi = R(T.c0) - R(0.5) * q2;
+  o.psi = pvd_exp<R>(o.lnpsi);
+#pragma unroll
+  for (int p = 0; p < NP; ++p) {
+    o.d1[p] = g[p];
+    o.d2[p] = g[p] * g[p] + h[p];
+  }
+}
+
+}  // namespace pvd
