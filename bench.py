@@ -95,6 +95,25 @@ def cpu_baseline(n):
                       f"CPU: {cpu_model()}"}
 
 
+def copy_bandwidth(nbytes=1 << 30, reps=10):
+    """Achievable HBM bandwidth on this box: a plain device-to-device copy of `nbytes`
+    (read + write = 2 * nbytes per copy), timed with HIP events."""
+    import torch
+    src = torch.empty(nbytes // 4, dtype=torch.float32, device='cuda').fill_(1.0)
+    dst = torch.empty_like(src)
+    dst.copy_(src)
+    torch.cuda.synchronize()
+    a, b = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    a.record()
+    for _ in range(reps):
+        dst.copy_(src)
+    b.record()
+    torch.cuda.synchronize()
+    ms = a.elapsed_time(b) / reps
+    del src, dst
+    return 2 * nbytes / (ms * 1e-3) / 1e9
+
+
 def load_traffic(key):
     path = os.path.join(ROOT, 'profiles', 'traffic.json')
     try:
@@ -114,6 +133,7 @@ def main():
     ap.add_argument('--no-cpu', action='store_true')
     ap.add_argument('--no-variants', action='store_true')
     ap.add_argument('--event-steps', type=int, default=200)
+    ap.add_argument('--copy-only', action='store_true', help='only the 1 GiB copy (counter calibration)')
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU: the DMC walkers of this workload live on a single MI355X "
@@ -122,6 +142,9 @@ def main():
         sys.exit(2)
     import torch
     assert torch.cuda.is_available(), 'bench.py needs the GPU'
+    if args.copy_only:
+        print(json.dumps({"copy_GBs": copy_bandwidth()}), flush=True)
+        return
     n = args.walkers
     K, W = args.steps, args.warmup
     total = W + K + args.event_steps + 2
@@ -146,6 +169,12 @@ def main():
     log(f'[bench] plain: {value:.4g} walker-steps/s, {ms_per_step * 1e3:.2f} us/step, kernel {kern_ms} ms')
     del sim
     torch.cuda.empty_cache()
+
+    copy_gbs = copy_bandwidth()
+    if roof is not None:
+        roof["copy_GBs"] = copy_gbs
+        roof["frac_of_copy"] = roof["achieved"] / copy_gbs
+    log(f'[bench] device copy: {copy_gbs:.0f} GB/s')
 
     variants = {}
     if not args.no_variants:

@@ -163,6 +163,50 @@ int pvd_trial_vjp(const pvd_trial_desc* trial, int32_t precision, int32_t n_atom
                   int32_t n_dim, const void* coords, int64_t n, const void* grad_out,
                   void* grad_coords, void* stream);
 
+/* --- surrogate potential: descriptor + MLP on BF16 MFMA / FP32 accumulation ---------
+ * Learned replacement of a user potential (the NN_Potential convention,
+ * potential_manager.py:177-214 and sample_potentials/TensorflowPots/call_sample_model.py:4-9),
+ * with the descriptor of simulation_utilities/tensorflow_descriptors/distance_descriptors.py
+ * (inverse pair distances sorted inside like-atom groups, :102-152).  Network:
+ * D -> H -> H -> H -> 1 with SiLU, H in {128, 256}, D <= 32; flat FP32 parameters in the
+ * order W0 (H x D), b0, W1 (H x H), b1, W2, b2, w3 (H), b3. */
+typedef struct pvd_mlp {
+  int32_t D, H;
+  int32_t npairs;          /* descriptor pairs, D <= npairs <= 15 */
+  int32_t pair_i[15], pair_j[15], pair_group[15];
+  int32_t use_range;       /* 1: walkers outside [lo, hi] use the fallback potential */
+  float xmean[32], xinv[32], lo[32], hi[32];
+  float ymean, ystd;
+  const float* params;     /* device, pvd_mlp_param_count(D, H) floats */
+  void* packs;             /* device, pvd_mlp_pack_bytes(H) bytes, written by pvd_mlp_pack */
+} pvd_mlp;
+
+const char* pvd_mlp_last_error(void);
+int64_t pvd_mlp_param_count(int32_t D, int32_t H);
+int64_t pvd_mlp_pack_bytes(int32_t H);
+int64_t pvd_mlp_workspace_bytes(int32_t D, int32_t H, int64_t batch);
+/* FP32 master parameters -> BF16 MFMA operand packs (forward and transposed). */
+int pvd_mlp_pack(const pvd_mlp* m, void* stream);
+/* Descriptors of AoS walkers (device, (n, n_atoms, 3)) into features_t (device, D x n),
+ * optionally normalised with xmean / xinv. */
+int pvd_mlp_descriptors(const pvd_mlp* m, int32_t precision, int32_t n_atoms, const void* coords, int64_t n,
+                        int32_t normalise, float* features_t, void* stream);
+/* V (device, float64) of walkers in SoA planes (component p of walker i at
+ * planes[p*stride + i]); the walker count is read from n_dev on the device when non-NULL,
+ * else n_max.  `fallback` is the analytic potential used outside the training range. */
+int pvd_mlp_energy(const pvd_mlp* m, int32_t precision, int32_t n_atoms, const void* planes, int64_t stride,
+                   const int32_t* n_dev, int64_t n_max, double* v_out, const pvd_pes_desc* fallback, void* stream);
+/* Mean-squared-error gradients (device, flat parameter order) on normalised features
+ * x_t (D x n) and targets (n); y_out (n) or NULL; loss_parts (ceil(n/256) doubles). */
+int pvd_mlp_grads(const pvd_mlp* m, const float* x_t, const float* targets, int64_t n, float* grads, float* y_out,
+                  double* loss_parts, void* workspace, void* stream);
+/* One Adam step (Kingma & Ba) with bias correction at step t >= 1. */
+int pvd_mlp_adam(float* params, float* m1, float* m2, const float* grads, int64_t size, float lr, float beta1,
+                 float beta2, float eps, int64_t t, void* stream);
+/* Device view of the current walkers without a host sync: SoA planes, stride, and a
+ * device pointer to the current population. */
+int pvd_sim_device_view(pvd_sim* sim, void** planes, int64_t* stride, int32_t** n_dev);
+
 #ifdef __cplusplus
 }
 #endif

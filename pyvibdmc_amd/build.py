@@ -1,8 +1,8 @@
 """Build recipe for the HIP library (gfx950).
 
-``python -m pyvibdmc_amd.build`` compiles ``csrc/pvd_capi.hip`` (which includes the kernel
-headers) into ``pyvibdmc_amd/libpyvibdmc_amd.so`` in place, and ``csrc/pvd_mlp.hip`` into
-``libpyvibdmc_amd_mlp.so``.
+``python -m pyvibdmc_amd.build`` compiles ``csrc/pvd_capi.hip`` (step kernels and the
+simulation C ABI) and ``csrc/pvd_mlp.hip`` (surrogate kernels) into
+``pyvibdmc_amd/libpyvibdmc_amd.so`` in place.
 """
 import os
 import subprocess
@@ -18,8 +18,7 @@ FLAGS = ['-O3', f'--offload-arch={ARCH}', '-std=c++17', '-fPIC', '-shared', '-ff
          '-Xclang', '-target-feature', '-Xclang', '-packed-fp32-ops']
 
 TARGETS = [
-    ('libpyvibdmc_amd.so', ['pvd_capi.hip']),
-    ('libpyvibdmc_amd_mlp.so', ['pvd_mlp.hip']),
+    ('libpyvibdmc_amd.so', ['pvd_capi.hip', 'pvd_mlp.hip']),
 ]
 
 

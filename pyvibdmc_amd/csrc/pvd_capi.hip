@@ -726,6 +726,15 @@ int pvd_sim_planes(pvd_sim* s, void** planes, int64_t* stride, int32_t* n, void*
   return 0;
 }
 
+int pvd_sim_device_view(pvd_sim* s, void** planes, int64_t* stride, int32_t** n_dev) {
+  if (!s) return fail(E_BAD, "null argument");
+  const int cb = cur_buf(s);
+  if (planes) *planes = s->coords[cb];
+  if (stride) *stride = s->cap;
+  if (n_dev) *n_dev = &s->ctrl->n[(int)(s->next_step & 1)];
+  return 0;
+}
+
 int pvd_sim_status(pvd_sim* s, int32_t* status, int64_t* fail_step, int32_t* n, double* vref,
                    int64_t* next_step, void* stream) {
   if (!s) return fail(E_BAD, "null argument");

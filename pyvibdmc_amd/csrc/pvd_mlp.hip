@@ -1,0 +1,693 @@
+// Surrogate potential: sorted inverse-distance descriptor + MLP (D -> H -> H -> H -> 1,
+// SiLU) on BF16 MFMA with FP32 accumulation, for inference inside the DMC step and for
+// training (forward with stored pre-activations, backward, weight gradients, Adam).
+//
+// MFMA formulation (v_mfma_f32_16x16x32_bf16, wave64).  Weights are the A operand
+// (16 output units x 32 inputs) and activations the B operand (32 inputs x 16 walkers);
+// lane l holds A[l%16][8(l/16)+j], B[8(l/16)+j][l%16] (j < 8) and D[4(l/16)+i][l%16]
+// (i < 4).  A layer's accumulators therefore already hold, for walker l%16, the units
+// 16m + 4(l/16) + i.  Reading the next layer's 32-wide input chunk c as
+//   k = 8g + j  ->  unit 16(2c + j/4) + 4g + j%4
+// makes those registers the next B operand directly, so activations never leave
+// registers; the weights are stored pre-permuted to match (pvd_mlp_pack).
+#include <hip/hip_runtime.h>
+
+#include <cstring>
+#include <string>
+
+#include "../../include/pyvibdmc_amd.h"
+#include "pvd_models.hpp"
+
+namespace pvd {
+
+typedef __attribute__((ext_vector_type(8))) __bf16 bf16x8;
+typedef __attribute__((ext_vector_type(4))) float f32x4;
+
+constexpr int MLP_MAXD = 32;
+constexpr int MLP_MAXP = 15;
+
+struct MlpNet {
+  int D, H;
+  const bf16x8* Wf[3];
+  const bf16x8* Wt[2];  // transposed packs of layers 1 and 2
+  const float* b[3];
+  const float* w3;
+  const float* b3;
+  int npairs, use_range;
+  int pi[MLP_MAXP], pj[MLP_MAXP], grp[MLP_MAXP];
+  float xmean[MLP_MAXD], xinv[MLP_MAXD], lo[MLP_MAXD], hi[MLP_MAXD];
+  float ymean, ystd;
+};
+
+PVD_DEV float silu_f(float z) { return z / (1.0f + __expf(-z)); }
+PVD_DEV float dsilu_f(float z) {
+  const float s = 1.0f / (1.0f + __expf(-z));
+  return s * (1.0f + z * (1.0f - s));
+}
+
+// unit index of element j of input chunk c for lane group g (the permuted K order)
+PVD_DEV int phys_unit(int c, int g, int j) { return 16 * (2 * c + (j >> 2)) + 4 * g + (j & 3); }
+
+// descriptor of one walker: inverse distances of the pairs, sorted descending inside each
+// group by odd-even transposition (comparisons never cross a group boundary)
+template <typename R, int NP>
+PVD_DEV void mlp_descriptor(const MlpNet& M, const R* x, float* f) {
+#pragma unroll
+  for (int k = 0; k < MLP_MAXP; ++k) {
+    f[k] = 0.0f;
+    if (k < M.npairs) {
+      const int bi = 3 * M.pi[k], bj = 3 * M.pj[k];
+      float d[3];
+#pragma unroll
+      for (int c = 0; c < 3; ++c) {
+        R xi = R(0), xj = R(0);
+#pragma unroll
+        for (int p = 0; p < NP; ++p) {
+          if (p == bi + c) xi = x[p];
+          if (p == bj + c) xj = x[p];
+        }
+        d[c] = (float)(xi - xj);
+      }
+      f[k] = 1.0f / __fsqrt_rn(d[0] * d[0] + d[1] * d[1] + d[2] * d[2]);
+    }
+  }
+#pragma unroll
+  for (int r = 0; r < MLP_MAXP; ++r) {
+#pragma unroll
+    for (int k = (r & 1); k + 1 < MLP_MAXP; k += 2) {
+      if (k + 1 < M.npairs && M.grp[k] == M.grp[k + 1] && f[k] < f[k + 1]) {
+        const float t = f[k];
+        f[k] = f[k + 1];
+        f[k + 1] = t;
+      }
+    }
+  }
+}
+
+// normalised features 8g..8g+7 of one walker as a B operand chunk
+PVD_DEV bf16x8 input_chunk(const MlpNet& M, const float* f, int g, bool valid) {
+  bf16x8 v;
+#pragma unroll
+  for (int j = 0; j < 8; ++j) {
+    float s = 0.0f;
+#pragma unroll
+    for (int k = 0; k < MLP_MAXD; ++k)
+      if (k == 8 * g + j && k < M.D) s = (f[k < MLP_MAXP ? k : 0] - M.xmean[k]) * M.xinv[k];
+    v[j] = (__bf16)(valid ? s : 0.0f);
+  }
+  return v;
+}
+
+// D(m) += sum_c A(m, c) B(c) for NT walker tiles; A read from LDS
+template <int NT, int MT, int KC>
+PVD_DEV void mfma_layer(const bf16x8* __restrict__ Wl, const bf16x8 (&bin)[NT][KC], f32x4 (&acc)[NT][MT], int lane) {
+#pragma unroll
+  for (int m = 0; m < MT; ++m) {
+#pragma unroll
+    for (int t = 0; t < NT; ++t) acc[t][m] = f32x4{0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+    for (int c = 0; c < KC; ++c) {
+      const bf16x8 a = Wl[(m * KC + c) * 64 + lane];
+#pragma unroll
+      for (int t = 0; t < NT; ++t) acc[t][m] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, bin[t][c], acc[t][m], 0, 0, 0);
+    }
+  }
+}
+
+template <int N>
+PVD_DEV void stage(bf16x8* lds, const bf16x8* __restrict__ src, int words) {
+  __syncthreads();
+  for (int q = threadIdx.x; q < words; q += N) lds[q] = src[q];
+  __syncthreads();
+}
+
+// ------------------------------------------------------------------------------------
+// Network forward.  MODE 0: inference on walkers (SoA planes of R) -> V (double), with the
+// analytic fallback outside the training range.  MODE 1: training forward on normalised
+// descriptors X^T (D x B) -> pre-activations Z^T (3 x H x B) and y (B).
+template <typename R, int NP, int H, int NT, int MODE>
+__global__ __launch_bounds__(256) void k_mlp_forward(MlpNet M, const R* planes, long long stride,
+                                                     const int* n_dev, long long n_fix, double* V, PesParams fb,
+                                                     const float* XT, float* ZT, float* Y) {
+  extern __shared__ bf16x8 lds[];
+  constexpr int MT = H / 16, KC = H / 32;
+  const int n = n_dev ? *n_dev : (int)n_fix;
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const int g = lane >> 4, col = lane & 15;
+  const long long base = ((long long)blockIdx.x * 4 + wave) * (16 * NT);
+  if ((long long)blockIdx.x * 4 * 16 * NT >= n) return;  // whole block past the end
+  bf16x8 bin0[NT][1];
+  bf16x8 bin[NT][KC];
+  bool fbk[NT];
+  double vfb[NT];
+#pragma unroll
+  for (int t = 0; t < NT; ++t) {
+    const long long w = base + t * 16 + col;
+    const bool ok = w < n;
+    fbk[t] = false;
+    vfb[t] = 0.0;
+    if constexpr (MODE == 0) {
+      float f[MLP_MAXP];
+      R x[NP];
+#pragma unroll
+      for (int p = 0; p < NP; ++p) x[p] = ok ? planes[p * stride + w] : R(0);
+      mlp_descriptor<R, NP>(M, x, f);
+      if (ok && M.use_range) {
+        bool out = false;
+#pragma unroll
+        for (int k = 0; k < MLP_MAXP; ++k)
+          if (k < M.D) out |= !(f[k] >= M.lo[k] && f[k] <= M.hi[k]);
+        if (out) {
+          fbk[t] = true;
+          vfb[t] = (double)pes_eval<R, NP>(fb, x);
+        }
+      }
+      bin0[t][0] = input_chunk(M, f, g, ok);
+    } else {
+      bf16x8 v;
+#pragma unroll
+      for (int j = 0; j < 8; ++j) {
+        const int k = 8 * g + j;
+        v[j] = (__bf16)((ok && k < M.D) ? XT[(long long)k * n + w] : 0.0f);
+      }
+      bin0[t][0] = v;
+    }
+  }
+  f32x4 acc[NT][MT];
+  float part[NT];
+  // layer 0 (input chunk c = 0 in natural order)
+  stage<256>(lds, M.Wf[0], MT * 64);
+  mfma_layer<NT, MT, 1>(lds, bin0, acc, lane);
+#pragma unroll
+  for (int layer = 0; layer < 3; ++layer) {
+    if (layer > 0) {
+      stage<256>(lds, M.Wf[layer], MT * KC * 64);
+      mfma_layer<NT, MT, KC>(lds, bin, acc, lane);
+    }
+    const float* bl = M.b[layer];
+#pragma unroll
+    for (int t = 0; t < NT; ++t) {
+      const long long w = base + t * 16 + col;
+      const bool ok = w < n;
+      if (layer == 2) part[t] = 0.0f;
+#pragma unroll
+      for (int m = 0; m < MT; ++m) {
+#pragma unroll
+        for (int i = 0; i < 4; ++i) {
+          const int u = 16 * m + 4 * g + i;
+          const float z = acc[t][m][i] + bl[u];
+          if constexpr (MODE == 1) {
+            if (ok) ZT[((long long)layer * H + u) * n + w] = z;
+          }
+          const float h = silu_f(z);
+          if (layer < 2) bin[t][m >> 1][(m & 1) * 4 + i] = (__bf16)h;
+          else part[t] += M.w3[u] * h;
+        }
+      }
+    }
+  }
+#pragma unroll
+  for (int t = 0; t < NT; ++t) {
+    float s = part[t];
+    s += __shfl_xor(s, 16, 64);
+    s += __shfl_xor(s, 32, 64);
+    const long long w = base + t * 16 + col;
+    if (g == 0 && w < n) {
+      const float y = s + M.b3[0];
+      if constexpr (MODE == 0) {
+        V[w] = fbk[t] ? vfb[t] : (double)(y * M.ystd + M.ymean);
+      } else {
+        Y[w] = y;
+      }
+    }
+  }
+}
+
+// Backward through the hidden layers for one batch: dZ_l^T = dL/dz_l (3 x H x B) from the
+// stored pre-activations and dy = dL/dy.  dh_{l-1} = W_l^T dz_l uses the transposed packs.
+template <int H, int NT>
+__global__ __launch_bounds__(256) void k_mlp_backward(MlpNet M, long long n, const float* ZT, const float* dY,
+                                                      float* DZT) {
+  extern __shared__ bf16x8 lds[];
+  constexpr int MT = H / 16, KC = H / 32;
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const int g = lane >> 4, col = lane & 15;
+  const long long base = ((long long)blockIdx.x * 4 + wave) * (16 * NT);
+  if ((long long)blockIdx.x * 4 * 16 * NT >= n) return;
+  bf16x8 bin[NT][KC];
+  f32x4 acc[NT][MT];
+  // layer 2: dz2 = w3 * dy * silu'(z2)
+#pragma unroll
+  for (int t = 0; t < NT; ++t) {
+    const long long w = base + t * 16 + col;
+    const bool ok = w < n;
+    const float dy = ok ? dY[w] : 0.0f;
+#pragma unroll
+    for (int m = 0; m < MT; ++m)
+#pragma unroll
+      for (int i = 0; i < 4; ++i) {
+        const int u = 16 * m + 4 * g + i;
+        const float z = ok ? ZT[((long long)2 * H + u) * n + w] : 0.0f;
+        const float dz = M.w3[u] * dy * dsilu_f(z);
+        if (ok) DZT[((long long)2 * H + u) * n + w] = dz;
+        bin[t][m >> 1][(m & 1) * 4 + i] = (__bf16)dz;
+      }
+  }
+#pragma unroll
+  for (int layer = 2; layer >= 1; --layer) {
+    stage<256>(lds, M.Wt[layer - 1], MT * KC * 64);
+    mfma_layer<NT, MT, KC>(lds, bin, acc, lane);
+#pragma unroll
+    for (int t = 0; t < NT; ++t) {
+      const long long w = base + t * 16 + col;
+      const bool ok = w < n;
+#pragma unroll
+      for (int m = 0; m < MT; ++m)
+#pragma unroll
+        for (int i = 0; i < 4; ++i) {
+          const int u = 16 * m + 4 * g + i;
+          const float z = ok ? ZT[((long long)(layer - 1) * H + u) * n + w] : 0.0f;
+          const float dz = acc[t][m][i] * dsilu_f(z);
+          if (ok) DZT[((long long)(layer - 1) * H + u) * n + w] = dz;
+          bin[t][m >> 1][(m & 1) * 4 + i] = (__bf16)dz;
+        }
+    }
+  }
+}
+
+// Weight gradient of layer l over a batch slice: P[split][o][i] = sum_b dz[o][b] * hin[i][b]
+// with hin = X (l = 0) or silu(z_{l-1}).  One wave per 32 x 32 output block.
+__global__ __launch_bounds__(64) void k_mlp_wgrad(int layer, int H, int Din, long long n, int kslice,
+                                                  const float* XT, const float* ZT, const float* DZT, float* P) {
+  const int lane = threadIdx.x;
+  const int g = lane >> 4, col = lane & 15;
+  const int ob = blockIdx.x * 32, ib = blockIdx.y * 32, split = blockIdx.z;
+  const long long k0 = (long long)split * kslice;
+  const long long k1 = k0 + kslice < n ? k0 + kslice : n;
+  const float* dz = DZT + (long long)layer * H * n;
+  const float* hz = layer == 0 ? XT : ZT + (long long)(layer - 1) * H * n;
+  f32x4 acc[2][2];
+#pragma unroll
+  for (int a = 0; a < 2; ++a)
+#pragma unroll
+    for (int b = 0; b < 2; ++b) acc[a][b] = f32x4{0.f, 0.f, 0.f, 0.f};
+  for (long long kb = k0; kb < k1; kb += 32) {
+    bf16x8 A[2], B[2];
+#pragma unroll
+    for (int a = 0; a < 2; ++a) {
+      const int o = ob + 16 * a + col;
+#pragma unroll
+      for (int j = 0; j < 8; ++j) {
+        const long long b = kb + 8 * g + j;
+        A[a][j] = (__bf16)((b < k1 && o < H) ? dz[(long long)o * n + b] : 0.0f);
+      }
+    }
+#pragma unroll
+    for (int c = 0; c < 2; ++c) {
+      const int i = ib + 16 * c + col;
+#pragma unroll
+      for (int j = 0; j < 8; ++j) {
+        const long long b = kb + 8 * g + j;
+        float h = 0.0f;
+        if (b < k1 && i < Din) {
+          const float v = hz[(long long)i * n + b];
+          h = layer == 0 ? v : silu_f(v);
+        }
+        B[c][j] = (__bf16)h;
+      }
+    }
+#pragma unroll
+    for (int a = 0; a < 2; ++a)
+#pragma unroll
+      for (int c = 0; c < 2; ++c) acc[a][c] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(A[a], B[c], acc[a][c], 0, 0, 0);
+  }
+  const int Hin = layer == 0 ? Din : H;
+#pragma unroll
+  for (int a = 0; a < 2; ++a)
+#pragma unroll
+    for (int c = 0; c < 2; ++c)
+#pragma unroll
+      for (int i4 = 0; i4 < 4; ++i4) {
+        const int o = ob + 16 * a + 4 * g + i4;
+        const int i = ib + 16 * c + col;
+        if (o < H && i < Hin) P[((long long)split * H + o) * Hin + i] = acc[a][c][i4];
+      }
+}
+
+// fixed-order sum of the split partials into the gradient vector
+__global__ void k_mlp_wgrad_reduce(const float* P, int splits, long long size, float* G) {
+  const long long q = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (q >= size) return;
+  float s = 0.0f;
+  for (int k = 0; k < splits; ++k) s += P[k * size + q];
+  G[q] = s;
+}
+
+// bias gradients and the output layer: one block per row, fixed-order reduction
+__global__ __launch_bounds__(256) void k_mlp_rowsum(int H, long long n, const float* ZT, const float* DZT,
+                                                    const float* dY, float* gb0, float* gb1, float* gb2, float* gw3,
+                                                    float* gb3) {
+  __shared__ float sh[4];
+  const int row = blockIdx.x;  // 0..3H: b0, b1, b2 rows, then w3 rows, then b3
+  float s = 0.0f;
+  for (long long b = threadIdx.x; b < n; b += 256) {
+    if (row < 3 * H) {
+      s += DZT[(long long)row * n + b];
+    } else if (row < 4 * H) {
+      const int u = row - 3 * H;
+      s += dY[b] * silu_f(ZT[((long long)2 * H + u) * n + b]);
+    } else {
+      s += dY[b];
+    }
+  }
+  s = block_sum<256>(s, sh);
+  if (threadIdx.x == 0) {
+    if (row < H) gb0[row] = s;
+    else if (row < 2 * H) gb1[row - H] = s;
+    else if (row < 3 * H) gb2[row - 2 * H] = s;
+    else if (row < 4 * H) gw3[row - 3 * H] = s;
+    else gb3[0] = s;
+  }
+}
+
+// residual of the MSE loss: dy = 2 (y - t) / n; per-block squared error into loss parts
+__global__ __launch_bounds__(256) void k_mlp_loss(long long n, const float* Y, const float* T, float* dY,
+                                                  double* parts) {
+  __shared__ double sh[4];
+  const long long b = (long long)blockIdx.x * 256 + threadIdx.x;
+  double e = 0.0;
+  if (b < n) {
+    const float r = Y[b] - T[b];
+    dY[b] = 2.0f * r / (float)n;
+    e = (double)r * (double)r;
+  }
+  e = block_sum<256>(e, sh);
+  if (threadIdx.x == 0) parts[blockIdx.x] = e;
+}
+
+// Adam with bias correction on the flat FP32 parameter vector
+__global__ void k_adam(float* p, float* m, float* v, const float* g, long long size, float lr, float b1, float b2,
+                       float eps, float c1, float c2) {
+  const long long q = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (q >= size) return;
+  const float gg = g[q];
+  const float mm = b1 * m[q] + (1.0f - b1) * gg;
+  const float vv = b2 * v[q] + (1.0f - b2) * gg * gg;
+  m[q] = mm;
+  v[q] = vv;
+  p[q] = p[q] - lr * (mm / c1) / (sqrtf(vv / c2) + eps);
+}
+
+// FP32 master weights -> packed BF16 operands.  Forward pack of layer l (A = W_l):
+// element (m, c, lane, j) = W[16m + lane%16][k], k = 8(lane/16)+j (layer 0) or the
+// permuted unit (layers 1, 2).  Transposed pack (A = W_l^T): W[permuted k][16m + lane%16].
+__global__ void k_mlp_pack(const float* W, int rows, int cols, int in_natural, int transpose, int MT, int KC,
+                           bf16x8* out) {
+  const long long q = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  const long long total = (long long)MT * KC * 64;
+  if (q >= total) return;
+  const int lane = (int)(q % 64);
+  const int c = (int)((q / 64) % KC);
+  const int m = (int)(q / (64 * KC));
+  const int g = lane >> 4, col = lane & 15;
+  bf16x8 v;
+#pragma unroll
+  for (int j = 0; j < 8; ++j) {
+    const int k = in_natural ? 32 * c + 8 * g + j : phys_unit(c, g, j);
+    const int r = 16 * m + col;
+    float x = 0.0f;
+    if (!transpose) {
+      if (r < rows && k < cols) x = W[(long long)r * cols + k];
+    } else {
+      // A = W^T: row r is an input unit of W, k an output unit
+      if (k < rows && r < cols) x = W[(long long)k * cols + r];
+    }
+    v[j] = (__bf16)x;
+  }
+  out[q] = v;
+}
+
+// descriptors of AoS walkers (n, M, 3) -> raw features F^T (D x n), float32
+template <typename R, int NP>
+__global__ void k_mlp_desc(MlpNet M, const R* x, long long n, float* FT) {
+  const long long w = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (w >= n) return;
+  R xx[NP];
+#pragma unroll
+  for (int p = 0; p < NP; ++p) xx[p] = x[w * NP + p];
+  float f[MLP_MAXP];
+  mlp_descriptor<R, NP>(M, xx, f);
+#pragma unroll
+  for (int k = 0; k < MLP_MAXP; ++k)
+    if (k < M.npairs) FT[(long long)k * n + w] = f[k];
+}
+
+// raw features -> normalised features in place
+__global__ void k_mlp_normalise(MlpNet M, long long n, float* FT) {
+  const long long q = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (q >= (long long)M.D * n) return;
+  const int k = (int)(q / n);
+  FT[q] = (FT[q] - M.xmean[k]) * M.xinv[k];
+}
+
+}  // namespace pvd
+
+using namespace pvd;
+
+namespace {
+thread_local std::string g_merr;
+int mfail(int code, const std::string& m) {
+  g_merr = m;
+  return code;
+}
+#define MHIP(x)                                                                                  \
+  do {                                                                                           \
+    hipError_t e_ = (x);                                                                         \
+    if (e_ != hipSuccess) return mfail(8, std::string(#x) + ": " + hipGetErrorString(e_));       \
+  } while (0)
+
+long long off_W(int l, int D, int H) {
+  // flat parameter order: W0 (H x D), b0, W1 (H x H), b1, W2, b2, w3, b3
+  switch (l) {
+    case 0: return 0;
+    case 1: return (long long)H * D + H;
+    default: return (long long)H * D + H + (long long)H * H + H;
+  }
+}
+long long off_b(int l, int D, int H) { return off_W(l, D, H) + (l == 0 ? (long long)H * D : (long long)H * H); }
+long long off_w3(int D, int H) { return (long long)H * D + 3ll * H + 2ll * H * H; }
+
+bool build_net(const pvd_mlp* m, MlpNet& N) {
+  std::memset(&N, 0, sizeof(N));
+  const int D = m->D, H = m->H;
+  if (D < 1 || D > 32 || (H != 128 && H != 256) || m->npairs > 15 || m->npairs < D) return false;
+  N.D = D;
+  N.H = H;
+  const float* p = (const float*)m->params;
+  for (int l = 0; l < 3; ++l) N.b[l] = p + off_b(l, D, H);
+  N.w3 = p + off_w3(D, H);
+  N.b3 = N.w3 + H;
+  const bf16x8* pk = (const bf16x8*)m->packs;
+  const long long s0 = (long long)(H / 16) * 64, s1 = (long long)(H / 16) * (H / 32) * 64;
+  N.Wf[0] = pk;
+  N.Wf[1] = pk + s0;
+  N.Wf[2] = pk + s0 + s1;
+  N.Wt[0] = pk + s0 + 2 * s1;
+  N.Wt[1] = pk + s0 + 3 * s1;
+  N.npairs = m->npairs;
+  N.use_range = m->use_range;
+  for (int k = 0; k < 15; ++k) {
+    N.pi[k] = m->pair_i[k];
+    N.pj[k] = m->pair_j[k];
+    N.grp[k] = m->pair_group[k];
+  }
+  for (int k = 0; k < 32; ++k) {
+    N.xmean[k] = m->xmean[k];
+    N.xinv[k] = m->xinv[k];
+    N.lo[k] = m->lo[k];
+    N.hi[k] = m->hi[k];
+  }
+  N.ymean = m->ymean;
+  N.ystd = m->ystd;
+  return true;
+}
+
+constexpr int NT_FWD = 2;
+
+template <typename R, int NP, int H>
+int launch_energy(const MlpNet& N, const void* planes, long long stride, const int* n_dev, long long n_max,
+                  double* V, const PesParams& fb, hipStream_t st) {
+  const long long per_block = 4ll * 16 * NT_FWD;
+  const unsigned grid = (unsigned)((n_max + per_block - 1) / per_block);
+  const size_t lds = (size_t)(H / 16) * (H / 32) * 64 * 16;
+  auto k = k_mlp_forward<R, NP, H, NT_FWD, 0>;
+  MHIP(hipFuncSetAttribute((const void*)k, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
+  hipLaunchKernelGGL(k, dim3(grid), dim3(256), lds, st, N, (const R*)planes, stride, n_dev, n_max, V, fb,
+                     nullptr, nullptr, nullptr);
+  MHIP(hipGetLastError());
+  return 0;
+}
+
+}  // namespace
+
+extern "C" {
+
+const char* pvd_mlp_last_error(void) { return g_merr.c_str(); }
+
+int64_t pvd_mlp_param_count(int32_t D, int32_t H) { return off_w3(D, H) + H + 1; }
+
+int64_t pvd_mlp_pack_bytes(int32_t H) {
+  return ((int64_t)(H / 16) * 64 + 4ll * (H / 16) * (H / 32) * 64) * 16;
+}
+
+int pvd_mlp_pack(const pvd_mlp* m, void* stream) {
+  if (!m || !m->params || !m->packs) return mfail(7, "null argument");
+  const int D = m->D, H = m->H;
+  if (H != 128 && H != 256) return mfail(7, "hidden width must be 128 or 256");
+  hipStream_t st = (hipStream_t)stream;
+  const float* p = (const float*)m->params;
+  bf16x8* pk = (bf16x8*)m->packs;
+  const int MT = H / 16, KC = H / 32;
+  const long long s0 = (long long)MT * 64, s1 = (long long)MT * KC * 64;
+  hipLaunchKernelGGL(k_mlp_pack, dim3((unsigned)((s0 + 255) / 256)), dim3(256), 0, st, p + off_W(0, D, H), H, D, 1, 0,
+                     MT, 1, pk);
+  hipLaunchKernelGGL(k_mlp_pack, dim3((unsigned)((s1 + 255) / 256)), dim3(256), 0, st, p + off_W(1, D, H), H, H, 0, 0,
+                     MT, KC, pk + s0);
+  hipLaunchKernelGGL(k_mlp_pack, dim3((unsigned)((s1 + 255) / 256)), dim3(256), 0, st, p + off_W(2, D, H), H, H, 0, 0,
+                     MT, KC, pk + s0 + s1);
+  hipLaunchKernelGGL(k_mlp_pack, dim3((unsigned)((s1 + 255) / 256)), dim3(256), 0, st, p + off_W(1, D, H), H, H, 0, 1,
+                     MT, KC, pk + s0 + 2 * s1);
+  hipLaunchKernelGGL(k_mlp_pack, dim3((unsigned)((s1 + 255) / 256)), dim3(256), 0, st, p + off_W(2, D, H), H, H, 0, 1,
+                     MT, KC, pk + s0 + 3 * s1);
+  MHIP(hipGetLastError());
+  return 0;
+}
+
+int pvd_mlp_descriptors(const pvd_mlp* m, int32_t precision, int32_t n_atoms, const void* coords, int64_t n,
+                        int32_t normalise, float* features_t, void* stream) {
+  MlpNet N;
+  if (!m || !coords || !features_t || !build_net(m, N)) return mfail(7, "bad network description");
+  if (n <= 0) return 0;
+  hipStream_t st = (hipStream_t)stream;
+  const unsigned grid = (unsigned)((n + 255) / 256);
+  const int np = n_atoms * 3;
+  if (precision == PVD_FP64) {
+    if (np == 9) hipLaunchKernelGGL((k_mlp_desc<double, 9>), dim3(grid), dim3(256), 0, st, N, (const double*)coords, n, features_t);
+    else if (np == 18) hipLaunchKernelGGL((k_mlp_desc<double, 18>), dim3(grid), dim3(256), 0, st, N, (const double*)coords, n, features_t);
+    else return mfail(7, "descriptor kernels support 3 or 6 atoms");
+  } else {
+    if (np == 9) hipLaunchKernelGGL((k_mlp_desc<float, 9>), dim3(grid), dim3(256), 0, st, N, (const float*)coords, n, features_t);
+    else if (np == 18) hipLaunchKernelGGL((k_mlp_desc<float, 18>), dim3(grid), dim3(256), 0, st, N, (const float*)coords, n, features_t);
+    else return mfail(7, "descriptor kernels support 3 or 6 atoms");
+  }
+  if (normalise)
+    hipLaunchKernelGGL(k_mlp_normalise, dim3((unsigned)((N.D * n + 255) / 256)), dim3(256), 0, st, N, (long long)n,
+                       features_t);
+  MHIP(hipGetLastError());
+  return 0;
+}
+
+int pvd_mlp_energy(const pvd_mlp* m, int32_t precision, int32_t n_atoms, const void* planes, int64_t stride,
+                   const int32_t* n_dev, int64_t n_max, double* v_out, const pvd_pes_desc* fallback, void* stream) {
+  MlpNet N;
+  if (!m || !planes || !v_out || !build_net(m, N)) return mfail(7, "bad network description");
+  if (n_max <= 0) return 0;
+  PesParams fb;
+  std::memset(&fb, 0, sizeof(fb));
+  if (fallback) std::memcpy(&fb, fallback, sizeof(fb));
+  if (N.use_range && !fallback) return mfail(7, "range check needs a fallback potential");
+  hipStream_t st = (hipStream_t)stream;
+  const int np = n_atoms * 3;
+#define PVD_E(R, NP, H) return launch_energy<R, NP, H>(N, planes, stride, n_dev, n_max, v_out, fb, st)
+  if (precision == PVD_FP64) {
+    if (np == 9 && N.H == 128) PVD_E(double, 9, 128);
+    if (np == 9 && N.H == 256) PVD_E(double, 9, 256);
+    if (np == 18 && N.H == 128) PVD_E(double, 18, 128);
+    if (np == 18 && N.H == 256) PVD_E(double, 18, 256);
+  } else {
+    if (np == 9 && N.H == 128) PVD_E(float, 9, 128);
+    if (np == 9 && N.H == 256) PVD_E(float, 9, 256);
+    if (np == 18 && N.H == 128) PVD_E(float, 18, 128);
+    if (np == 18 && N.H == 256) PVD_E(float, 18, 256);
+  }
+#undef PVD_E
+  return mfail(7, "surrogate supports 3 or 6 atoms and hidden width 128 or 256");
+}
+
+// Gradients of the mean squared error of the network on (X, T) into grads (flat, the
+// parameter order), plus the loss.  ws: device workspace of pvd_mlp_workspace_bytes().
+int64_t pvd_mlp_workspace_bytes(int32_t D, int32_t H, int64_t batch) {
+  const int64_t n = batch;
+  const int64_t splits = (n + 255) / 256;
+  const int64_t P = splits * (int64_t)H * (H > D ? H : D);
+  return (3 * H * n * 2 + 2 * n + P + (n + 255) / 256 * 2) * 4 + 64;
+}
+
+int pvd_mlp_grads(const pvd_mlp* m, const float* x_t, const float* targets, int64_t n, float* grads, float* y_out,
+                  double* loss_parts, void* workspace, void* stream) {
+  MlpNet N;
+  if (!m || !x_t || !targets || !grads || !workspace || !build_net(m, N)) return mfail(7, "bad arguments");
+  if (n <= 0) return mfail(7, "empty batch");
+  hipStream_t st = (hipStream_t)stream;
+  const int D = N.D, H = N.H;
+  float* ZT = (float*)workspace;
+  float* DZT = ZT + 3ll * H * n;
+  float* Y = DZT + 3ll * H * n;
+  float* dY = Y + n;
+  float* P = dY + n;
+  const long long per_block = 4ll * 16 * NT_FWD;
+  const unsigned grid = (unsigned)((n + per_block - 1) / per_block);
+  const size_t lds = (size_t)(H / 16) * (H / 32) * 64 * 16;
+  PesParams fb;
+  std::memset(&fb, 0, sizeof(fb));
+  if (H == 128) {
+    auto kf = k_mlp_forward<float, 9, 128, NT_FWD, 1>;
+    auto kb = k_mlp_backward<128, NT_FWD>;
+    MHIP(hipFuncSetAttribute((const void*)kf, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
+    MHIP(hipFuncSetAttribute((const void*)kb, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
+    hipLaunchKernelGGL(kf, dim3(grid), dim3(256), lds, st, N, (const float*)nullptr, 0ll, (const int*)nullptr,
+                       (long long)n, (double*)nullptr, fb, x_t, ZT, Y);
+    hipLaunchKernelGGL(k_mlp_loss, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, (long long)n, Y, targets, dY,
+                       loss_parts);
+    hipLaunchKernelGGL(kb, dim3(grid), dim3(256), lds, st, N, (long long)n, ZT, dY, DZT);
+  } else {
+    auto kf = k_mlp_forward<float, 9, 256, 1, 1>;
+    auto kb = k_mlp_backward<256, 1>;
+    const unsigned grid1 = (unsigned)((n + 63) / 64);
+    MHIP(hipFuncSetAttribute((const void*)kf, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
+    MHIP(hipFuncSetAttribute((const void*)kb, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
+    hipLaunchKernelGGL(kf, dim3(grid1), dim3(256), lds, st, N, (const float*)nullptr, 0ll, (const int*)nullptr,
+                       (long long)n, (double*)nullptr, fb, x_t, ZT, Y);
+    hipLaunchKernelGGL(k_mlp_loss, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, (long long)n, Y, targets, dY,
+                       loss_parts);
+    hipLaunchKernelGGL(kb, dim3(grid1), dim3(256), lds, st, N, (long long)n, ZT, dY, DZT);
+  }
+  const int kslice = 256;
+  const int splits = (int)((n + kslice - 1) / kslice);
+  for (int l = 0; l < 3; ++l) {
+    const int Din = l == 0 ? D : H;
+    dim3 g3((unsigned)(H / 32), (unsigned)((Din + 31) / 32), (unsigned)splits);
+    hipLaunchKernelGGL(k_mlp_wgrad, g3, dim3(64), 0, st, l, H, Din, (long long)n, kslice, x_t, ZT, DZT, P);
+    const long long size = (long long)H * Din;
+    hipLaunchKernelGGL(k_mlp_wgrad_reduce, dim3((unsigned)((size + 255) / 256)), dim3(256), 0, st, P, splits, size,
+                       grads + off_W(l, D, H));
+  }
+  hipLaunchKernelGGL(k_mlp_rowsum, dim3((unsigned)(4 * H + 1)), dim3(256), 0, st, H, (long long)n, ZT, DZT, dY,
+                     grads + off_b(0, D, H), grads + off_b(1, D, H), grads + off_b(2, D, H), grads + off_w3(D, H),
+                     grads + off_w3(D, H) + H);
+  if (y_out) MHIP(hipMemcpyAsync(y_out, Y, sizeof(float) * n, hipMemcpyDeviceToDevice, st));
+  MHIP(hipGetLastError());
+  return 0;
+}
+
+int pvd_mlp_adam(float* params, float* m1, float* m2, const float* grads, int64_t size, float lr, float beta1,
+                 float beta2, float eps, int64_t t, void* stream) {
+  if (!params || !m1 || !m2 || !grads || size <= 0 || t <= 0) return mfail(7, "bad arguments");
+  const float c1 = 1.0f - powf(beta1, (float)t), c2 = 1.0f - powf(beta2, (float)t);
+  hipLaunchKernelGGL(k_adam, dim3((unsigned)((size + 255) / 256)), dim3(256), 0, (hipStream_t)stream, params, m1, m2,
+                     grads, (long long)size, lr, beta1, beta2, eps, c1, c2);
+  MHIP(hipGetLastError());
+  return 0;
+}
+
+}  // extern "C"

@@ -349,8 +349,12 @@ class DMC_Sim:
 
     def _start(self):
         lib = N.load()
+        dev_e = getattr(self.potential_obj, 'device_energy', None)
         if self._fused:
             N.check(lib.pvd_sim_start(self._h, N.stream_handle()), 'pvd_sim_start')
+        elif dev_e is not None:
+            v = dev_e(self)
+            N.check(lib.pvd_sim_start_external(self._h, N.ptr(v), N.stream_handle()), 'pvd_sim_start_external')
         else:
             c, _ = self._get_walkers()
             v = self._dev(self._host_energy(c))
@@ -380,8 +384,12 @@ class DMC_Sim:
                 N.check(lib.pvd_sim_run(self._h, 1, N.stream_handle()), 'pvd_sim_run')
         else:
             N.check(lib.pvd_sim_move(self._h, N.ptr(disp), N.ptr(um), N.stream_handle()), 'pvd_sim_move')
-            c, _ = self._get_walkers()
-            v = self._dev(self._host_energy(c))
+            dev_e = getattr(self.potential_obj, 'device_energy', None)
+            if dev_e is not None:
+                v = dev_e(self)
+            else:
+                c, _ = self._get_walkers()
+                v = self._dev(self._host_energy(c))
             if draws is not None and self.weighting == 'discrete' and branching:
                 ub = self._dev(draws.uniform(n))
             N.check(lib.pvd_sim_branch_external(self._h, N.ptr(v), N.ptr(ub), N.stream_handle()),
@@ -390,8 +398,23 @@ class DMC_Sim:
 
     def _run_chunk(self, t0, t1):
         draws = self._draws()
+        dev_e = getattr(self.potential_obj, 'device_energy', None)
+        lib = N.load()
         if draws is None and self._fused:
-            N.check(N.load().pvd_sim_run(self._h, t1 - t0, N.stream_handle()), 'pvd_sim_run')
+            N.check(lib.pvd_sim_run(self._h, t1 - t0, N.stream_handle()), 'pvd_sim_run')
+            self._raise_if_failed()
+        elif draws is None and dev_e is not None:
+            # device potential outside the fused kernel (surrogate): move, V on the device,
+            # branch - no host synchronisation inside the chunk
+            every = getattr(self.potential_obj, 'judge_every', 0) or 0
+            st = N.stream_handle()
+            for t in range(t0, t1):
+                N.check(lib.pvd_sim_move(self._h, None, None, st), 'pvd_sim_move')
+                v = dev_e(self)
+                N.check(lib.pvd_sim_branch_external(self._h, N.ptr(v), None, st), 'pvd_sim_branch_external')
+                if every and (t + 1) % every == 0:
+                    self._raise_if_failed()
+                    self.potential_obj.judge(self, t)
             self._raise_if_failed()
         else:
             for t in range(t0, t1):

@@ -1,0 +1,316 @@
+"""Learned surrogate potential: sorted inverse-distance descriptor + MLP, trained and
+evaluated by the library's BF16-MFMA kernels.
+
+The reference's learned-PES convention is ``NN_Potential`` (a model handed to a user
+function, simulation_utilities/potential_manager.py:177-214) with a pair-distance descriptor
+(simulation_utilities/tensorflow_descriptors/distance_descriptors.py:102-152) and training
+data dumped by ``DEBUG_save_training_every`` (pyvibdmc.py:800-804, 840-844).  Here the
+descriptor, forward pass, backward pass and Adam all run on the GPU; ``SurrogatePotential``
+plugs into ``DMC_Sim`` and is evaluated on the device straight from the walker planes,
+with the analytic potential as the fallback outside the training range and as a judge on a
+sample of walkers.
+"""
+import ctypes
+import time
+
+import numpy as np
+
+from . import _native as N
+from .potentials import DevicePotential
+
+__all__ = ['MLPDesc', 'pair_groups', 'SurrogateModel', 'SurrogatePotential']
+
+
+class MLPDesc(ctypes.Structure):
+    _fields_ = [('D', ctypes.c_int32), ('H', ctypes.c_int32), ('npairs', ctypes.c_int32),
+                ('pair_i', ctypes.c_int32 * 15), ('pair_j', ctypes.c_int32 * 15), ('pair_group', ctypes.c_int32 * 15),
+                ('use_range', ctypes.c_int32),
+                ('xmean', ctypes.c_float * 32), ('xinv', ctypes.c_float * 32), ('lo', ctypes.c_float * 32),
+                ('hi', ctypes.c_float * 32), ('ymean', ctypes.c_float), ('ystd', ctypes.c_float),
+                ('params', ctypes.c_void_p), ('packs', ctypes.c_void_p)]
+
+
+_P = ctypes.c_void_p
+_MSIGS = {
+    'pvd_mlp_last_error': (ctypes.c_char_p, []),
+    'pvd_mlp_param_count': (ctypes.c_int64, [ctypes.c_int32, ctypes.c_int32]),
+    'pvd_mlp_pack_bytes': (ctypes.c_int64, [ctypes.c_int32]),
+    'pvd_mlp_workspace_bytes': (ctypes.c_int64, [ctypes.c_int32, ctypes.c_int32, ctypes.c_int64]),
+    'pvd_mlp_pack': (ctypes.c_int, [ctypes.POINTER(MLPDesc), _P]),
+    'pvd_mlp_descriptors': (ctypes.c_int, [ctypes.POINTER(MLPDesc), ctypes.c_int32, ctypes.c_int32, _P,
+                                           ctypes.c_int64, ctypes.c_int32, _P, _P]),
+    'pvd_mlp_energy': (ctypes.c_int, [ctypes.POINTER(MLPDesc), ctypes.c_int32, ctypes.c_int32, _P, ctypes.c_int64,
+                                      _P, ctypes.c_int64, _P, ctypes.POINTER(N.PesDesc), _P]),
+    'pvd_mlp_grads': (ctypes.c_int, [ctypes.POINTER(MLPDesc), _P, _P, ctypes.c_int64, _P, _P, _P, _P, _P]),
+    'pvd_mlp_adam': (ctypes.c_int, [_P, _P, _P, _P, ctypes.c_int64, ctypes.c_float, ctypes.c_float,
+                                    ctypes.c_float, ctypes.c_float, ctypes.c_int64, _P]),
+    'pvd_sim_device_view': (ctypes.c_int, [_P, ctypes.POINTER(_P), ctypes.POINTER(ctypes.c_int64),
+                                           ctypes.POINTER(_P)]),
+}
+
+
+def _lib():
+    lib = N.load()
+    if not getattr(lib, '_mlp_bound', False):
+        for name, (res, args) in _MSIGS.items():
+            f = getattr(lib, name)
+            f.restype = res
+            f.argtypes = args
+        lib._mlp_bound = True
+    return lib
+
+
+def _check(rc, what):
+    if rc != 0:
+        raise RuntimeError(f'{what}: {_lib().pvd_mlp_last_error().decode()}')
+
+
+def pair_groups(elements):
+    """All atom pairs (i < j), grouped by unordered element pair in order of first
+    appearance; returns (pairs, group id per pair).  Within a group the descriptor is
+    sorted, which makes it invariant to permutations of like atoms."""
+    pairs = [(i, j) for i in range(len(elements)) for j in range(i + 1, len(elements))]
+    keys = []
+    for i, j in pairs:
+        k = tuple(sorted((elements[i], elements[j])))
+        if k not in keys:
+            keys.append(k)
+    out, grp = [], []
+    for gi, k in enumerate(keys):
+        for (i, j) in pairs:
+            if tuple(sorted((elements[i], elements[j]))) == k:
+                out.append((i, j))
+                grp.append(gi)
+    return out, grp
+
+
+class SurrogateModel:
+    """D -> H -> H -> H -> 1 SiLU network on the sorted inverse pair distances of
+    ``elements`` (H = 128 or 256), with FP32 master parameters and BF16 MFMA packs on the GPU."""
+
+    def __init__(self, elements, hidden=128, seed=0):
+        import torch
+        self.elements = list(elements)
+        pairs, grp = pair_groups(self.elements)
+        if len(pairs) > 15:
+            raise ValueError('at most 15 atom pairs (6 atoms)')
+        self.n_atoms = len(self.elements)
+        self.D = len(pairs)
+        self.H = hidden
+        lib = _lib()
+        d = MLPDesc()
+        d.D, d.H, d.npairs = self.D, hidden, len(pairs)
+        for k, ((i, j), g) in enumerate(zip(pairs, grp)):
+            d.pair_i[k], d.pair_j[k], d.pair_group[k] = i, j, g
+        for k in range(32):
+            d.xmean[k] = 0.0
+            d.xinv[k] = 1.0
+            d.lo[k] = -1e30
+            d.hi[k] = 1e30
+        d.ymean, d.ystd = 0.0, 1.0
+        self.desc = d
+        npar = lib.pvd_mlp_param_count(self.D, hidden)
+        rs = np.random.RandomState(seed)
+        H, D = hidden, self.D
+
+        def lin(o, i):
+            return (rs.uniform(-1, 1, (o, i)) * np.sqrt(6.0 / (i + o))).astype(np.float32)
+        flat = np.concatenate([lin(H, D).ravel(), np.zeros(H, np.float32), lin(H, H).ravel(), np.zeros(H, np.float32),
+                               lin(H, H).ravel(), np.zeros(H, np.float32),
+                               (rs.uniform(-1, 1, H) * np.sqrt(6.0 / (H + 1))).astype(np.float32),
+                               np.zeros(1, np.float32)])
+        assert flat.size == npar
+        self.params = torch.from_numpy(flat).cuda()
+        self.m1 = torch.zeros_like(self.params)
+        self.m2 = torch.zeros_like(self.params)
+        self.grads = torch.zeros_like(self.params)
+        self.packs = torch.empty(int(lib.pvd_mlp_pack_bytes(hidden)), dtype=torch.uint8, device='cuda')
+        d.params = self.params.data_ptr()
+        d.packs = self.packs.data_ptr()
+        self.t = 0
+        self.history = []
+        self.repack()
+
+    # -- parameters as NumPy views (for checks) --------------------------------------
+    def param_dict(self):
+        p = self.params.cpu().numpy()
+        H, D = self.H, self.D
+        o = 0
+        out = {}
+        for k, sz, sh in (('W0', H * D, (H, D)), ('b0', H, (H,)), ('W1', H * H, (H, H)), ('b1', H, (H,)),
+                          ('W2', H * H, (H, H)), ('b2', H, (H,)), ('w3', H, (H,)), ('b3', 1, (1,))):
+            out[k] = p[o:o + sz].reshape(sh).copy()
+            o += sz
+        return out
+
+    def set_params(self, pd):
+        import torch
+        flat = np.concatenate([np.asarray(pd[k], np.float32).ravel() for k in ('W0', 'b0', 'W1', 'b1', 'W2', 'b2',
+                                                                            'w3', 'b3')])
+        self.params.copy_(torch.from_numpy(flat).cuda())
+        self.repack()
+
+    def repack(self):
+        _check(_lib().pvd_mlp_pack(ctypes.byref(self.desc), N.stream_handle()), 'pvd_mlp_pack')
+
+    # -- descriptors --------------------------------------------------------------------
+    def features(self, coords, normalise=True):
+        """Descriptors of (n, M, 3) walkers (NumPy or device tensor) as a device tensor (D, n)."""
+        import torch
+        x = coords if isinstance(coords, torch.Tensor) else torch.from_numpy(np.ascontiguousarray(coords, np.float64))
+        x = x.cuda().contiguous()
+        prec = N.PVD_FP64 if x.dtype == torch.float64 else N.PVD_FP32
+        n = x.shape[0]
+        ft = torch.empty((self.D, n), dtype=torch.float32, device='cuda')
+        _check(_lib().pvd_mlp_descriptors(ctypes.byref(self.desc), prec, self.n_atoms, x.data_ptr(), n,
+                                          1 if normalise else 0, ft.data_ptr(), N.stream_handle()),
+               'pvd_mlp_descriptors')
+        return ft
+
+    def set_normalisation(self, coords, energies, range_margin=0.0):
+        import torch
+        f = self.features(coords, normalise=False)
+        mean = f.mean(dim=1).cpu().numpy()
+        std = f.std(dim=1).cpu().numpy()
+        lo = f.min(dim=1).values.cpu().numpy()
+        hi = f.max(dim=1).values.cpu().numpy()
+        e = np.asarray(energies, np.float64)
+        d = self.desc
+        for k in range(self.D):
+            d.xmean[k] = float(mean[k])
+            d.xinv[k] = float(1.0 / max(std[k], 1e-12))
+            span = hi[k] - lo[k]
+            d.lo[k] = float(lo[k] - range_margin * span)
+            d.hi[k] = float(hi[k] + range_margin * span)
+        d.ymean = float(e.mean())
+        d.ystd = float(e.std())
+
+    # -- training ----------------------------------------------------------------------
+    def _workspace(self, n):
+        import torch
+        nb = int(_lib().pvd_mlp_workspace_bytes(self.D, self.H, n))
+        if getattr(self, '_ws', None) is None or self._ws.numel() < nb:
+            self._ws = torch.empty(nb, dtype=torch.uint8, device='cuda')
+            self._lp = torch.empty((n + 255) // 256 + 1, dtype=torch.float64, device='cuda')
+        return self._ws, self._lp
+
+    def loss_and_grads(self, x_t, targets):
+        """MSE on normalised features (D, n) and normalised targets (n); fills self.grads."""
+        import torch
+        n = targets.shape[0]
+        ws, lp = self._workspace(n)
+        y = torch.empty(n, dtype=torch.float32, device='cuda')
+        _check(_lib().pvd_mlp_grads(ctypes.byref(self.desc), x_t.data_ptr(), targets.data_ptr(), n,
+                                    self.grads.data_ptr(), y.data_ptr(), lp.data_ptr(), ws.data_ptr(),
+                                    N.stream_handle()), 'pvd_mlp_grads')
+        return lp[:(n + 255) // 256].sum() / n, y
+
+    def adam_step(self, lr=1e-3, b1=0.9, b2=0.999, eps=1e-8):
+        self.t += 1
+        _check(_lib().pvd_mlp_adam(self.params.data_ptr(), self.m1.data_ptr(), self.m2.data_ptr(),
+                                   self.grads.data_ptr(), self.params.numel(), lr, b1, b2, eps, self.t,
+                                   N.stream_handle()), 'pvd_mlp_adam')
+        self.repack()
+
+    def fit(self, coords, energies, epochs=200, batch=8192, lr=1e-3, seed=0, verbose=False, max_seconds=None):
+        """Adam on the mean squared error of the normalised energies.  Data: (N, M, 3) walkers
+        and (N,) energies in hartree.  Returns the per-epoch training loss (normalised units)."""
+        import torch
+        self.set_normalisation(coords, energies)
+        x = self.features(coords, normalise=True)
+        e = torch.from_numpy(np.asarray(energies, np.float64)).cuda()
+        t = ((e - self.desc.ymean) / self.desc.ystd).float()
+        n = t.shape[0]
+        g = torch.Generator(device='cuda')
+        g.manual_seed(seed)
+        start = time.time()
+        losses = []
+        for ep in range(epochs):
+            perm = torch.randperm(n, device='cuda', generator=g)
+            tot = torch.zeros((), dtype=torch.float64, device='cuda')
+            nb = 0
+            for b0 in range(0, n - batch + 1 if n >= batch else 1, batch):
+                idx = perm[b0:b0 + batch]
+                xb = x[:, idx].contiguous()
+                tb = t[idx].contiguous()
+                loss, _ = self.loss_and_grads(xb, tb)
+                self.adam_step(lr)
+                tot += loss
+                nb += 1
+            losses.append(float(tot / max(nb, 1)))
+            if verbose and (ep % 10 == 0 or ep == epochs - 1):
+                print(f'epoch {ep}: loss {losses[-1]:.3e}', flush=True)
+            if max_seconds and time.time() - start > max_seconds:
+                break
+        self.history += losses
+        self.train_seconds = time.time() - start
+        return losses
+
+    # -- inference ----------------------------------------------------------------------
+    def energy_planes(self, planes_ptr, stride, n_dev_ptr, n_max, out, precision, fallback=None):
+        fb = fallback.desc if fallback is not None else None
+        _check(_lib().pvd_mlp_energy(ctypes.byref(self.desc), precision, self.n_atoms, planes_ptr, stride,
+                                     n_dev_ptr, n_max, out.data_ptr(), ctypes.byref(fb) if fb is not None else None,
+                                     N.stream_handle()), 'pvd_mlp_energy')
+
+    def energy(self, coords, fallback=None):
+        """V (hartree, float64 NumPy) of (n, M, 3) host walkers."""
+        import torch
+        x = torch.from_numpy(np.ascontiguousarray(coords, np.float64)).cuda()
+        n = x.shape[0]
+        planes = x.reshape(n, -1).t().contiguous()
+        out = torch.empty(n, dtype=torch.float64, device='cuda')
+        self.energy_planes(planes.data_ptr(), n, None, n, out, N.PVD_FP64, fallback)
+        return out.cpu().numpy()
+
+
+class SurrogatePotential(DevicePotential):
+    """A trained ``SurrogateModel`` as a DMC potential.  Walkers whose descriptor leaves the
+    training range use ``fallback`` (an analytic device potential); every ``judge_every``
+    steps the analytic potential is evaluated on every ``judge_stride``-th walker and the
+    surrogate's RMS and maximum error there are recorded in ``judge_log``."""
+
+    def __init__(self, model, fallback, use_range=True, judge_every=100, judge_stride=100):
+        super().__init__()
+        self.model = model
+        self.fallback = fallback
+        self.n_atoms = model.n_atoms
+        self.n_dim = 3
+        self.desc.kind = N.PES_EXTERNAL
+        self.model.desc.use_range = 1 if use_range else 0
+        self.judge_every = judge_every
+        self.judge_stride = judge_stride
+        self.judge_log = []
+        self._V = None
+
+    def getpot(self, cds, timeit=False):
+        start = time.time() if timeit else None
+        v = self.model.energy(cds, self.fallback)
+        return (v, time.time() - start) if timeit else v
+
+    def device_energy(self, sim):
+        """V of the sim's current walkers into a persistent device buffer (no host sync)."""
+        import torch
+        lib = _lib()
+        planes = ctypes.c_void_p()
+        stride = ctypes.c_int64()
+        ndev = ctypes.c_void_p()
+        N.check(lib.pvd_sim_device_view(sim._h, ctypes.byref(planes), ctypes.byref(stride), ctypes.byref(ndev)),
+                'pvd_sim_device_view')
+        cap = stride.value
+        if self._V is None or self._V.numel() < cap:
+            self._V = torch.empty(cap, dtype=torch.float64, device='cuda')
+        prec = N.PVD_FP64 if sim.precision == 'fp64' else N.PVD_FP32
+        self.model.energy_planes(planes.value, cap, ndev.value, cap, self._V, prec, self.fallback)
+        return self._V
+
+    def judge(self, sim, step):
+        """RMS / max |V_surrogate - V_analytic| (hartree) on every judge_stride-th walker."""
+        c, _ = sim._get_walkers()
+        sub = c[::self.judge_stride]
+        va = self.fallback.getpot(sub)
+        vs = self.model.energy(sub, self.fallback if self.model.desc.use_range else None)
+        err = vs - va
+        rec = (step, float(np.sqrt(np.mean(err ** 2))), float(np.max(np.abs(err))), len(sub))
+        self.judge_log.append(rec)
+        return rec

@@ -1,0 +1,114 @@
+"""GPU surrogate kernels against the CPU oracle (oracle/mlp.py).
+
+Bars: descriptors to float32 rounding (1e-6 relative); forward energies and parameter
+gradients to the BF16 operand precision (2e-2 of the tensor norm: both sides round the same
+operands to bfloat16, and accumulation order flips a few roundings); training reduces the
+loss and reaches sub-milli-hartree accuracy on the triatomic surface.
+"""
+import numpy as np
+import pytest
+
+from oracle import mlp as O
+from oracle import pes as P
+
+pytestmark = pytest.mark.gpu
+
+
+def water_walkers(n, seed, noise=0.08):
+    return P.water_eq()[None] + np.random.RandomState(seed).normal(0, noise, (n, 3, 3))
+
+
+def oracle_net(model):
+    net = O.MLP(model.D, model.H)
+    net.p = {k: v.astype(np.float32) for k, v in model.param_dict().items()}
+    return net
+
+
+def normalised_features(model, x):
+    d = model.desc
+    pairs, sizes = O.pair_groups(model.elements)
+    f = O.descriptor(x, pairs, sizes)
+    mean = np.array([d.xmean[k] for k in range(model.D)], np.float32)
+    inv = np.array([d.xinv[k] for k in range(model.D)], np.float32)
+    return ((f.astype(np.float32) - mean) * inv).astype(np.float32)
+
+
+@pytest.mark.parametrize('elements,shape', [(['H', 'H', 'O'], 3), (['O', 'H', 'H', 'O', 'H', 'H'], 6)])
+def test_descriptors_match_oracle(elements, shape):
+    import pyvibdmc_amd as pv
+    m = pv.SurrogateModel(elements, hidden=128, seed=1)
+    if shape == 3:
+        x = water_walkers(1000, 0)
+    else:
+        x = P.cluster6_eq()[None] + np.random.RandomState(2).normal(0, 0.1, (1000, 6, 3))
+    got = m.features(x, normalise=False).cpu().numpy().T
+    pairs, sizes = O.pair_groups(elements)
+    np.testing.assert_allclose(got, O.descriptor(x, pairs, sizes), rtol=2e-6)
+
+
+@pytest.mark.parametrize('hidden', [128, 256])
+def test_forward_matches_oracle(hidden):
+    import pyvibdmc_amd as pv
+    m = pv.SurrogateModel(['H', 'H', 'O'], hidden=hidden, seed=3)
+    x = water_walkers(3000, 4)
+    m.set_normalisation(x, P.morse_bend(x))
+    v = m.energy(x)
+    y, _ = oracle_net(m).forward(normalised_features(m, x))
+    ref = y * m.desc.ystd + m.desc.ymean
+    err = np.linalg.norm(v - ref) / np.linalg.norm(ref - ref.mean())
+    assert err < 2e-2, err
+
+
+@pytest.mark.parametrize('hidden', [128, 256])
+def test_gradients_match_oracle(hidden):
+    import torch
+    import pyvibdmc_amd as pv
+    m = pv.SurrogateModel(['H', 'H', 'O'], hidden=hidden, seed=5)
+    x = water_walkers(2048, 6)
+    e = P.morse_bend(x)
+    m.set_normalisation(x, e)
+    xt = m.features(x, normalise=True)
+    t = torch.from_numpy(((e - m.desc.ymean) / m.desc.ystd).astype(np.float32)).cuda()
+    loss, y = m.loss_and_grads(xt, t)
+    gd = torch.clone(m.grads).cpu().numpy()
+    net = oracle_net(m)
+    lo, yo, go = net.loss_grads(xt.cpu().numpy().T.copy(), t.cpu().numpy().astype(np.float64))
+    assert abs(float(loss) - lo) <= 2e-2 * lo
+    H, D = m.H, m.D
+    o = 0
+    for k, sz in (('W0', H * D), ('b0', H), ('W1', H * H), ('b1', H), ('W2', H * H), ('b2', H), ('w3', H), ('b3', 1)):
+        a = gd[o:o + sz]
+        b = go[k].reshape(-1)
+        o += sz
+        rel = np.linalg.norm(a - b) / max(np.linalg.norm(b), 1e-30)
+        assert rel < 3e-2, (k, rel)
+
+
+def test_training_reduces_loss_and_learns_the_surface():
+    import pyvibdmc_amd as pv
+    m = pv.SurrogateModel(['H', 'H', 'O'], hidden=128, seed=7)
+    x = water_walkers(16384, 8, noise=0.1)
+    e = P.morse_bend(x)
+    losses = m.fit(x, e, epochs=40, batch=1024, lr=2e-3, seed=0)
+    assert losses[-1] < 0.05 * losses[0]
+    xv = water_walkers(4000, 9, noise=0.1)
+    rms = np.sqrt(np.mean((m.energy(xv) - P.morse_bend(xv)) ** 2))
+    assert rms < 5e-4, rms  # hartree
+
+
+def test_adam_matches_oracle_step():
+    import pyvibdmc_amd as pv
+    m = pv.SurrogateModel(['H', 'H', 'O'], hidden=128, seed=11)
+    p0 = m.param_dict()
+    g = np.random.RandomState(0).normal(0, 1, m.params.numel()).astype(np.float32)
+    import torch
+    m.grads.copy_(torch.from_numpy(g).cuda())
+    m.adam_step(lr=1e-3)
+    m.grads.copy_(torch.from_numpy(0.5 * g).cuda())
+    m.adam_step(lr=1e-3)
+    flat0 = np.concatenate([p0[k].ravel() for k in ('W0', 'b0', 'W1', 'b1', 'W2', 'b2', 'w3', 'b3')])
+    pp = {'x': flat0.copy()}
+    opt = O.Adam(pp, lr=1e-3)
+    opt.step(pp, {'x': g})
+    opt.step(pp, {'x': 0.5 * g})
+    np.testing.assert_allclose(m.params.cpu().numpy(), pp['x'], rtol=1e-5, atol=1e-7)
