@@ -1,0 +1,151 @@
+"""Stand-alone device potentials and trial functions against the oracle, their hand-written
+backward kernels against oracle gradients and FP64 central differences, and statistical
+checks of the production (Philox) path."""
+import numpy as np
+import pytest
+
+from oracle import pes as P
+
+pytestmark = pytest.mark.gpu
+
+
+def water(n, seed, noise=0.08):
+    return P.water_eq()[None] + np.random.RandomState(seed).normal(0, noise, (n, 3, 3))
+
+
+def cluster(n, seed, noise=0.08):
+    return P.cluster6_eq()[None] + np.random.RandomState(seed).normal(0, noise, (n, 6, 3))
+
+
+def cases():
+    import pyvibdmc_amd as pv
+    k = P.M_H * P.HO_OMEGA ** 2
+    x1 = np.random.RandomState(1).normal(0, 0.2, (500, 1, 3))
+    return [
+        ('ho_x', pv.HarmonicOscillator1D(k=k), lambda c: P.harmonic_x(c, k), None, x1),
+        ('morse_bend', pv.MorseBendTriatomic(), P.morse_bend, P.morse_bend_grad, water(2000, 2)),
+        ('cluster6', pv.TwoMonomerCluster(), P.cluster6, P.cluster6_grad, cluster(2000, 3)),
+    ]
+
+
+@pytest.mark.parametrize('i', range(3))
+def test_device_pes_matches_oracle_fp64_and_fp32(i):
+    import torch
+    name, dev, ref, _, x = cases()[i]
+    v = dev.getpot(x)
+    np.testing.assert_allclose(v, ref(x), rtol=1e-12, atol=1e-15)
+    x32 = torch.from_numpy(x.astype(np.float32)).cuda()
+    from pyvibdmc_amd import _native as N
+    v32 = dev.eval_device(x32, precision=N.PVD_FP32).cpu().numpy()
+    np.testing.assert_allclose(v32, ref(x.astype(np.float32).astype(np.float64)), rtol=2e-5, atol=2e-6)
+
+
+@pytest.mark.parametrize('i', [1, 2])
+def test_pes_backward_kernel_matches_gradients(i):
+    """PotentialEnergy autograd: backward kernel vs oracle analytic gradient and FP64 central differences."""
+    import torch
+    from pyvibdmc_amd.impsamp import PotentialEnergy
+    name, dev, ref, grad, x = cases()[i]
+    xt = torch.from_numpy(x[:200]).cuda().requires_grad_(True)
+    v = PotentialEnergy.apply(xt, dev)
+    v.sum().backward()
+    g = xt.grad.cpu().numpy()
+    np.testing.assert_allclose(g, grad(x[:200]), rtol=1e-9, atol=1e-12)
+    h = 1e-4
+    fd = np.zeros_like(x[:20])
+    for a in range(x.shape[1]):
+        for d in range(3):
+            xp = x[:20].copy()
+            xm = x[:20].copy()
+            xp[:, a, d] += h
+            xm[:, a, d] -= h
+            fd[:, a, d] = (dev.getpot(xp) - dev.getpot(xm)) / (2 * h)
+    np.testing.assert_allclose(g[:20], fd, rtol=1e-5, atol=1e-8)
+    np.testing.assert_allclose(dev.forces(x[:20]), -g[:20], rtol=1e-12, atol=1e-15)
+
+
+@pytest.mark.parametrize('which', ['water', 'cluster', 'ho1d'])
+def test_trial_kernels_match_oracle(which):
+    import torch
+    import pyvibdmc_amd as pv
+    from pyvibdmc_amd.impsamp import TrialLogPsi
+    if which == 'water':
+        dev, ref, x = pv.water_trial(), P.water_trial(), water(1000, 4)
+        dev.desc.c0 = 0.0
+    elif which == 'cluster':
+        dev, ref, x = pv.cluster6_trial(), P.cluster6_trial(), cluster(1000, 5)
+    else:
+        dev, ref, x = pv.HarmonicTrial1D(), P.HarmonicTrial1D(), np.random.RandomState(6).normal(0, 0.1, (500, 1, 1))
+    d1, d2 = dev.call_derivs(x)
+    r1, r2 = ref.derivs(x)
+    np.testing.assert_allclose(d1, r1, rtol=1e-10, atol=1e-10)
+    np.testing.assert_allclose(d2, r2, rtol=1e-9, atol=1e-8)
+    if which != 'ho1d':
+        # ln psi forward / backward kernel against the oracle and central differences of ln psi
+        xt = torch.from_numpy(x[:100]).cuda().requires_grad_(True)
+        lp = TrialLogPsi.apply(xt, dev)
+        np.testing.assert_allclose(lp.detach().cpu().numpy(), ref.log_trial(x[:100]) - ref.log_trial(x[:1])[0]
+                                   + lp.detach().cpu().numpy()[0], rtol=1e-10, atol=1e-10)
+        lp.sum().backward()
+        np.testing.assert_allclose(xt.grad.cpu().numpy(), r1[:100], rtol=1e-10, atol=1e-10)
+        h = 1e-4
+        fd = np.zeros_like(x[:10])
+        for a in range(x.shape[1]):
+            for d in range(3):
+                xp = x[:10].copy()
+                xm = x[:10].copy()
+                xp[:, a, d] += h
+                xm[:, a, d] -= h
+                fd[:, a, d] = (ref.log_trial(xp) - ref.log_trial(xm)) / (2 * h)
+        np.testing.assert_allclose(r1[:10], fd, rtol=1e-6, atol=1e-7)
+
+
+def test_c0_philox_zpe_matches_discrete_tau_energy():
+    """Production path, 1-D oscillator (config 0): 5000 walkers, dt = 10, 2000 steps, FP64.
+    The mean vref over the last 3/4 matches the exact discrete-tau ground state 1498.83 cm^-1."""
+    import pyvibdmc_amd as pv
+    from oracle.eigen import discrete_tau_energy
+    k = P.M_H * P.HO_OMEGA ** 2
+    exact = discrete_tau_energy(P.M_H, k, 10.0) / P.WAVENUMBERS
+    zs = []
+    for seed in range(4):
+        sim = pv.DMC_Sim(num_walkers=5000, num_timesteps=2000, atoms=['X'], masses=P.M_H, delta_t=10,
+                         potential=pv.HarmonicOscillator1D(k=k), start_structures=np.zeros((1, 1, 3)),
+                         precision='fp64', seed=seed, write_files=False, equil_steps=10 ** 9, wfn_every=10 ** 9,
+                         chkpt_every=10 ** 9)
+        sim.propagate()
+        zs.append(sim.zpe())
+        pop = sim._pop_vs_tau[500:]
+        assert 4000 < pop.mean() < 5300
+    zs = np.array(zs)
+    err = zs.std(ddof=1) / np.sqrt(len(zs))
+    assert abs(zs.mean() - exact) < max(4 * err, 3.0), (zs, exact)
+
+
+def test_massive_death_raises_reference_error():
+    """A potential far above vref kills every walker: the reference's ValueError."""
+    import pyvibdmc_amd as pv
+    sim = pv.DMC_Sim(num_walkers=1000, num_timesteps=50, atoms=['X'], masses=P.M_H, delta_t=10,
+                     potential=pv.HarmonicOscillator1D(k=1.0), start_structures=np.full((1, 1, 3), 3.0),
+                     precision='fp64', seed=0, write_files=False, equil_steps=10 ** 9, wfn_every=10 ** 9,
+                     chkpt_every=10 ** 9, DEBUG_alpha=1e6)
+    with pytest.raises(ValueError, match='Massive walker birth or death event'):
+        sim.propagate()
+
+
+def test_restart_from_checkpoint_continues_the_same_draws(tmp_path):
+    """A restart from the step-100 checkpoint reproduces the uninterrupted run: Philox draws
+    are addressed by (walker slot, step), and the checkpoint keeps slot order."""
+    import pyvibdmc_amd as pv
+    kw = dict(num_walkers=2000, atoms=['H', 'H', 'O'], masses=[1837.15, 1837.15, 29156.95], delta_t=5,
+              start_structures=water(2000, 7, 0.03), precision='fp32', seed=42, equil_steps=10 ** 9,
+              wfn_every=10 ** 9, output_folder=str(tmp_path))
+    a = pv.DMC_Sim(sim_name='a', num_timesteps=200, chkpt_every=100, potential=pv.MorseBendTriatomic(), **kw)
+    a.propagate()                      # leaves the step-100 checkpoint on disk
+    b = pv.dmc_restart(pv.MorseBendTriatomic(), str(tmp_path), 'a', additional_timesteps=100)
+    assert b.cur_timestep == 100
+    b.propagate()
+    c = pv.DMC_Sim(sim_name='c', num_timesteps=300, chkpt_every=10 ** 9, potential=pv.MorseBendTriatomic(), **kw)
+    c.propagate()
+    np.testing.assert_array_equal(b._pop_vs_tau, c._pop_vs_tau)
+    np.testing.assert_allclose(b._vref_vs_tau, c._vref_vs_tau, rtol=0, atol=1e-12)
