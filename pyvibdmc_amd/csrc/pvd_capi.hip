@@ -5,6 +5,7 @@
 
 #include <cmath>
 #include <cstdio>
+#include <cstdlib>
 #include <cstring>
 #include <string>
 
@@ -272,6 +273,8 @@ struct pvd_sim {
   long long next_step;
   unsigned epoch_next;
   int status_seen;
+  int use_graphs;
+  hipGraphExec_t graph[2];  // replayed step graphs, without / with descendant tracking
 };
 
 namespace {
@@ -332,6 +335,7 @@ StepArgs<R> make_args(pvd_sim* s) {
   A.branch = (s->next_step % be) == 0 ? 1 : 0;
   A.ms = s->M;
   A.ds = s->d;
+  A.is_ = s->is;
   return A;
 }
 
@@ -345,35 +349,128 @@ StepArgs<R> make_args(pvd_sim* s) {
     default: return fail(E_BAD, "fused kernels support 1, 3, 9 or 18 components per walker"); \
   }
 
+// The kernels of one fused step, in launch order, for the current parity.  Used both for
+// direct launches and as the node list of a replayed graph.
+struct Launch {
+  const void* fn;
+  dim3 grid, block;
+};
+
 template <typename R, bool INJ>
-int launch_fused_step(pvd_sim* s, const StepArgs<R>& A, hipStream_t st) {
+int step_kernels(pvd_sim* s, Launch* L, int& nl) {
   const unsigned g = nblk(s->cap, TILE);
+  nl = 0;
   const int pk = s->cfg.pes.kind;
   if (pk == PES_EXTERNAL) return fail(E_BAD, "fused step needs a device potential");
+  auto add = [&](const void* fn, unsigned grid, unsigned block) { L[nl++] = Launch{fn, dim3(grid), dim3(block)}; };
+#define PVD_ADD(...) PVD_NP_SWITCH(s->np, add((const void*)(__VA_ARGS__), g, TILE))
+  if (s->is) {
+    PVD_ADD(k_accept<R, NP, INJ>);
+    add((const void*)k_acc_finalize<R>, 1, FN_NT);
+  }
   if (!s->cont) {
     if (s->is) {
-      PVD_NP_SWITCH(s->np, hipLaunchKernelGGL((k_accept<R, NP, INJ>), dim3(g), dim3(TILE), 0, st, A);
-                            hipLaunchKernelGGL((k_discrete<R, NP, INJ, true, false>), dim3(g), dim3(TILE), 0, st, A));
+      PVD_NP_SWITCH(s->np, add((const void*)k_discrete<R, NP, INJ, true, false>,
+                               nblk(s->cap, TILE * disc_wpt<R, NP, false>()), TILE));
+      add((const void*)k_finalize_discrete<R, true>, 1, FN_NT);
     } else {
-      PVD_NP_SWITCH(s->np, hipLaunchKernelGGL((k_discrete<R, NP, INJ, false, false>), dim3(g), dim3(TILE), 0, st, A));
+      PVD_NP_SWITCH(s->np, add((const void*)k_discrete<R, NP, INJ, false, false>,
+                               nblk(s->cap, TILE * disc_wpt<R, NP, false>()), TILE));
+      add((const void*)k_finalize_discrete<R, false>, 1, FN_NT);
     }
   } else {
     if (s->is) {
-      PVD_NP_SWITCH(s->np, hipLaunchKernelGGL((k_accept<R, NP, INJ>), dim3(g), dim3(TILE), 0, st, A);
-                            hipLaunchKernelGGL((k_cont<R, NP, INJ, true, false>), dim3(g), dim3(TILE), 0, st, A));
+      PVD_ADD(k_cont<R, NP, INJ, true, false>);
     } else {
-      PVD_NP_SWITCH(s->np, hipLaunchKernelGGL((k_cont<R, NP, INJ, false, false>), dim3(g), dim3(TILE), 0, st, A));
+      PVD_ADD(k_cont<R, NP, INJ, false, false>);
     }
-    hipLaunchKernelGGL((k_cont_replay<R>), dim3(1), dim3(RP_NT), 0, st, A);
+    add((const void*)k_cont_replay<R>, 1, RP_NT);
   }
-  HIPCHK(hipGetLastError());
+#undef PVD_ADD
   return 0;
+}
+
+template <typename R, bool INJ>
+int launch_fused_step(pvd_sim* s, StepArgs<R>& A, hipStream_t st) {
+  Launch L[8];
+  int nl = 0;
+  int rc = step_kernels<R, INJ>(s, L, nl);
+  if (rc) return rc;
+  void* args[] = {&A};
+  for (int i = 0; i < nl; ++i) HIPCHK(hipLaunchKernel(L[i].fn, L[i].grid, L[i].block, args, 0, st));
+  return 0;
+}
+
+// A graph of PVD_GRAPH_STEPS consecutive steps starting at an even step: the kernels are
+// parameter-identical from one even/odd pair to the next (all step-dependent state is read
+// from the device control block), so one instantiated graph replays any number of times.
+constexpr int PVD_GRAPH_STEPS = 64;
+
+template <typename R>
+int build_graph(pvd_sim* s, hipGraphExec_t* out) {
+  hipGraph_t g;
+  HIPCHK(hipGraphCreate(&g, 0));
+  const long long saved = s->next_step;
+  const long long base = saved & ~1ll;
+  hipGraphNode_t prev = nullptr;
+  int rc = 0;
+  for (int k = 0; k < PVD_GRAPH_STEPS && rc == 0; ++k) {
+    s->next_step = base + k;
+    StepArgs<R> A = make_args<R>(s);
+    Launch L[8];
+    int nl = 0;
+    rc = step_kernels<R, false>(s, L, nl);
+    for (int i = 0; i < nl && rc == 0; ++i) {
+      hipKernelNodeParams p;
+      std::memset(&p, 0, sizeof(p));
+      void* args[] = {&A};
+      p.func = const_cast<void*>(L[i].fn);
+      p.gridDim = L[i].grid;
+      p.blockDim = L[i].block;
+      p.sharedMemBytes = 0;
+      p.kernelParams = args;
+      p.extra = nullptr;
+      hipGraphNode_t node;
+      hipError_t e = hipGraphAddKernelNode(&node, g, prev ? &prev : nullptr, prev ? 1 : 0, &p);
+      if (e != hipSuccess) rc = fail(E_HIP, std::string("hipGraphAddKernelNode: ") + hipGetErrorString(e));
+      prev = node;
+    }
+  }
+  s->next_step = saved;
+  if (rc == 0) {
+    hipError_t e = hipGraphInstantiate(out, g, nullptr, nullptr, 0);
+    if (e != hipSuccess) rc = fail(E_HIP, std::string("hipGraphInstantiate: ") + hipGetErrorString(e));
+  }
+  (void)hipGraphDestroy(g);
+  return rc;
 }
 
 template <typename R>
 int run_steps(pvd_sim* s, int n_steps, const double* disp, const double* um, const double* ub,
               hipStream_t st) {
-  for (int k = 0; k < n_steps; ++k) {
+  int k = 0;
+  const bool graphs = !disp && s->use_graphs && (s->cfg.branch_every <= 1) && n_steps >= PVD_GRAPH_STEPS + 1;
+  if (graphs) {
+    if (s->next_step & 1) {
+      StepArgs<R> A = make_args<R>(s);
+      int rc = launch_fused_step<R, false>(s, A, st);
+      if (rc) return rc;
+      s->next_step += 1;
+      s->epoch_next += 1;
+      ++k;
+    }
+    const int slot = s->who_on ? 1 : 0;
+    if (!s->graph[slot]) {
+      int rc = build_graph<R>(s, &s->graph[slot]);
+      if (rc) return rc;
+    }
+    for (; n_steps - k >= PVD_GRAPH_STEPS; k += PVD_GRAPH_STEPS) {
+      HIPCHK(hipGraphLaunch(s->graph[slot], st));
+      s->next_step += PVD_GRAPH_STEPS;
+      s->epoch_next += PVD_GRAPH_STEPS;
+    }
+  }
+  for (; k < n_steps; ++k) {
     StepArgs<R> A = make_args<R>(s);
     int rc;
     if (disp) {
@@ -405,6 +502,7 @@ int do_move(pvd_sim* s, const double* disp, const double* um, hipStream_t st) {
     } else {
       PVD_NP_SWITCH(s->np, hipLaunchKernelGGL((k_move<R, NP, false, true>), dim3(g), dim3(TILE), 0, st, A));
     }
+    hipLaunchKernelGGL((k_acc_finalize<R>), dim3(1), dim3(FN_NT), 0, st, A);
   } else {
     if (disp) hipLaunchKernelGGL((k_move<R, 1, true, false>), dim3(g), dim3(TILE), 0, st, A);
     else hipLaunchKernelGGL((k_move<R, 1, false, false>), dim3(g), dim3(TILE), 0, st, A);
@@ -418,7 +516,7 @@ int do_branch_ext(pvd_sim* s, const double* v, const double* ub, hipStream_t st)
   StepArgs<R> A = make_args<R>(s);
   A.V_ext = v;
   A.inj_ubr = ub;
-  const unsigned g = nblk(s->cap, TILE);
+  const unsigned g = s->cont ? nblk(s->cap, TILE) : nblk(s->cap, TILE * disc_wpt<R, 1, true>());
   if (!s->cont) {
     if (ub) {
       if (s->is) hipLaunchKernelGGL((k_discrete<R, 1, true, true, true>), dim3(g), dim3(TILE), 0, st, A);
@@ -427,6 +525,8 @@ int do_branch_ext(pvd_sim* s, const double* v, const double* ub, hipStream_t st)
       if (s->is) hipLaunchKernelGGL((k_discrete<R, 1, false, true, true>), dim3(g), dim3(TILE), 0, st, A);
       else hipLaunchKernelGGL((k_discrete<R, 1, false, false, true>), dim3(g), dim3(TILE), 0, st, A);
     }
+    if (s->is) hipLaunchKernelGGL((k_finalize_discrete<R, true>), dim3(1), dim3(FN_NT), 0, st, A);
+    else hipLaunchKernelGGL((k_finalize_discrete<R, false>), dim3(1), dim3(FN_NT), 0, st, A);
   } else {
     if (s->is) hipLaunchKernelGGL((k_cont<R, 1, false, true, true>), dim3(g), dim3(TILE), 0, st, A);
     else hipLaunchKernelGGL((k_cont<R, 1, false, false, true>), dim3(g), dim3(TILE), 0, st, A);
@@ -500,6 +600,8 @@ int trial_eval_t(const pvd_trial_desc* tr, int np, const void* x, long long n, v
 
 void free_sim(pvd_sim* s) {
   if (!s) return;
+  for (int i = 0; i < 2; ++i)
+    if (s->graph[i]) (void)hipGraphExecDestroy(s->graph[i]);
   void* ptrs[] = {s->coords[0], s->coords[1], s->who[0], s->who[1], s->w, s->E, s->ke, s->kmask,
                   s->ctrl, s->flags, s->rec, s->tr_vref, s->tr_pop, s->tr_dteff, s->tr_births,
                   s->tr_deaths, s->pend, s->fin, s->klist, s->killtag, s->rstage, s->stage, s->parent, s->parent_w, s->desc,
@@ -597,6 +699,10 @@ int pvd_sim_create(const pvd_config* cfg, pvd_sim** out) {
     return fail(E_HIP, std::string("init: ") + hipGetErrorString(e));
   }
   s->epoch_next = 1;
+  {
+    const char* e = std::getenv("PVD_GRAPHS");
+    s->use_graphs = !(e && e[0] == '0');
+  }
   *out = s;
   return 0;
 }

@@ -32,8 +32,11 @@ PVD_DEV Philox4 philox4x32_10(uint32_t c0, uint32_t c1, uint32_t c2, uint32_t c3
   const uint32_t W0 = 0x9E3779B9u, W1 = 0xBB67AE85u;
 #pragma unroll
   for (int r = 0; r < 10; ++r) {
-    const uint32_t hi0 = mulhi32(M0, c0), lo0 = M0 * c0;
-    const uint32_t hi1 = mulhi32(M1, c2), lo1 = M1 * c2;
+    // one 32x32 -> 64-bit product gives both halves (v_mad_u64_u32)
+    const uint64_t p0 = (uint64_t)M0 * (uint64_t)c0;
+    const uint64_t p1 = (uint64_t)M1 * (uint64_t)c2;
+    const uint32_t hi0 = (uint32_t)(p0 >> 32), lo0 = (uint32_t)p0;
+    const uint32_t hi1 = (uint32_t)(p1 >> 32), lo1 = (uint32_t)p1;
     const uint32_t n0 = hi1 ^ c1 ^ k0;
     const uint32_t n2 = hi0 ^ c3 ^ k1;
     c0 = n0;
@@ -52,7 +55,7 @@ PVD_DEV Philox4 philox4x32_10(uint32_t c0, uint32_t c1, uint32_t c2, uint32_t c3
 }
 
 // purposes of a draw (third counter word, low byte)
-enum : uint32_t { RNG_DISP = 1u, RNG_BRANCH = 2u, RNG_METRO = 3u };
+enum : uint32_t { RNG_DISP = 1u, RNG_BRANCH = 2u, RNG_METRO = 3u, RNG_STEP = 4u };
 
 struct Rng {
   uint32_t k0, k1;
@@ -128,6 +131,18 @@ struct Normals<double, NP> {
     }
   }
 };
+
+// NW consecutive 32-bit words of one walker's stream at one step (blocks 0, 1, ...)
+template <int NW>
+PVD_DEV void philox_words(const Rng& g, uint32_t slot, uint64_t step, uint32_t* w) {
+#pragma unroll
+  for (int b = 0; b < (NW + 3) / 4; ++b) {
+    const Philox4 r = g.block(slot, step, RNG_STEP, (uint32_t)b);
+#pragma unroll
+    for (int j = 0; j < 4; ++j)
+      if (4 * b + j < NW) w[4 * b + j] = r.v[j];
+  }
+}
 
 PVD_DEV double uniform_d(const Rng& g, uint32_t slot, uint64_t step, uint32_t purpose) {
   const Philox4 r = g.block(slot, step, purpose, 0u);
@@ -232,6 +247,18 @@ PVD_DEV unsigned fetch_add_u(unsigned* p, unsigned v) {
   return __hip_atomic_fetch_add(p, v, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_AGENT);
 }
 PVD_DEV void fence_acquire() { __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent"); }
+// Relaxed agent-scope atomics: coherent across the XCDs (they bypass the per-XCD L2) and,
+// unlike acquire / release, they need no L2 write-back or invalidate.  Used for values that
+// are self-contained in one 64-bit word (look-back flags carry their payload).
+PVD_DEV uint64_t ld_relaxed(const uint64_t* p) {
+  return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+PVD_DEV void st_relaxed(uint64_t* p, uint64_t v) {
+  __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+PVD_DEV unsigned fetch_add_relaxed(unsigned* p, unsigned v) {
+  return __hip_atomic_fetch_add(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
 PVD_DEV void fence_release() { __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent"); }
 
 }  // namespace pvd

@@ -7,10 +7,14 @@
 // + Box-Muller, or injected draws), evaluates the PES in registers, forms the
 // branching weight and copy count, and writes its survivors to their final slots in
 // np.repeat order.  The output offset of a tile comes from a single-pass decoupled
-// look-back over per-tile flags; the tile that comes last in that order reduces the
-// per-tile partial sums in a fixed order and writes vref and the population for the
-// next step.  All control values are double-buffered by step parity `par`, which is
-// a launch argument, so a launch never reads a value that the same launch writes.
+// look-back over per-tile flags.  A flag is one 64-bit word (epoch | status | prefix)
+// written and read with relaxed agent-scope atomics, so publishing it needs no fence:
+// on CDNA an agent-scope release writes back the whole L2 of the XCD, which per tile
+// would cost more than the step itself.  The per-tile partial sums are plain stores that
+// a one-workgroup finalise launch reduces in a fixed order (the kernel boundary makes
+// them visible), writing vref and the population of the next step.  All control values
+// are double-buffered by step parity `par`, which is a launch argument, so a launch never
+// reads a value that the same launch writes.
 #pragma once
 #include "pvd_models.hpp"
 
@@ -45,6 +49,7 @@ struct Ctrl {
   double eff_t;
   unsigned list_n;
   unsigned pad1;
+  int ntd[2];  // tiles of the discrete step with parity p
 };
 
 struct TileRec {
@@ -104,6 +109,7 @@ struct StepArgs {
   int par;
   int branch;
   int ms, ds;  // atoms, dims
+  int is_;     // importance sampling on
 };
 
 // ---------------------------------------------------------------------------- helpers
@@ -113,17 +119,41 @@ PVD_DEV void load_walker(const StepArgs<R>& A, long long i, R* x) {
   for (int p = 0; p < NP; ++p) x[p] = A.cin[p * A.cap + i];
 }
 
-// displacement draws for walker i: disp = sigma*z (Philox) or the injected values
-template <typename R, int NP, bool INJ>
-PVD_DEV void draw_disp(const StepArgs<R>& A, long long i, unsigned long long step, R* disp) {
+// All draws of walker slot i at one step: the displacement disp = sigma * z, the
+// Metropolis uniform and the branching uniform.  Injected values are read from the
+// caller's arrays (the reference's draw order).  Philox, FP32: one stream of words per
+// (slot, step) - Box-Muller pairs from consecutive words, then one 24-bit word per
+// uniform (9 normals + 2 uniforms = 3 Philox blocks).  Philox, FP64: 53-bit uniforms from
+// separate blocks per purpose.
+template <typename R, int NP, bool INJ, bool IS>
+PVD_DEV void draw_step(const StepArgs<R>& A, long long i, unsigned long long step, R* disp, double& um,
+                       double& ub, bool want_branch) {
   if constexpr (INJ) {
 #pragma unroll
     for (int p = 0; p < NP; ++p) disp[p] = (R)A.inj_disp[i * NP + p];
+    um = IS ? A.inj_umet[i] : 0.0;
+    ub = want_branch ? A.inj_ubr[i] : 0.0;
+  } else if constexpr (sizeof(R) == 4) {
+    constexpr int NPAIR = (NP + 1) / 2;
+    constexpr int NW = 2 * NPAIR + (IS ? 1 : 0) + 1;
+    uint32_t w[NW];
+    philox_words<NW>(A.rng, (uint32_t)i, step, w);
+#pragma unroll
+    for (int k = 0; k < NPAIR; ++k) {
+      float z0, z1;
+      box_muller_f(w[2 * k], w[2 * k + 1], z0, z1);
+      disp[2 * k] = (R)((float)A.sigma[2 * k] * z0);
+      if (2 * k + 1 < NP) disp[2 * k + 1] = (R)((float)A.sigma[2 * k + 1] * z1);
+    }
+    um = IS ? (double)u01f(w[2 * NPAIR]) : 0.0;
+    ub = (double)u01f(w[NW - 1]);
   } else {
     R z[NP];
     Normals<R, NP>::draw(A.rng, (uint32_t)i, step, z);
 #pragma unroll
     for (int p = 0; p < NP; ++p) disp[p] = (R)A.sigma[p] * z[p];
+    um = IS ? uniform_d(A.rng, (uint32_t)i, step, RNG_METRO) : 0.0;
+    ub = want_branch ? uniform_d(A.rng, (uint32_t)i, step, RNG_BRANCH) : 0.0;
   }
 }
 
@@ -160,11 +190,10 @@ PVD_DEV R local_kin(const StepArgs<R>& A, const R* sec) {
 // On return x holds the new position and the function returns the local energy's
 // kinetic part at that position; `acc` tells whether the proposal was accepted.
 template <typename R, int NP, bool INJ>
-PVD_DEV R is_move(const StepArgs<R>& A, long long i, unsigned long long step, R* x, bool& acc) {
+PVD_DEV R is_move(const StepArgs<R>& A, const R* disp, double u, R* x, bool& acc) {
   TrialOut<R, NP> tx, ty;
   trial_eval<R, NP>(A.trial, x, tx);
-  R disp[NP], y[NP], Dx[NP], Dy[NP];
-  draw_disp<R, NP, INJ>(A, i, step, disp);
+  R y[NP], Dx[NP], Dy[NP];
   const R dt = (R)A.dt;
 #pragma unroll
   for (int p = 0; p < NP; ++p) {
@@ -220,7 +249,6 @@ PVD_DEV R is_move(const StepArgs<R>& A, long long i, unsigned long long step, R*
     }
     met = pvd_exp<R>(la);
   }
-  const double u = draw_u<R, INJ>(A, i, step, RNG_METRO);
   acc = (double)met > u;
   if (acc) {
 #pragma unroll
@@ -250,7 +278,7 @@ PVD_DEV int look_back(uint64_t* flags, int tile, unsigned epoch, Ctrl* C, bool& 
     if (t >= 0) {
       long long spins = 0;
       while (true) {
-        f = ld_acquire(&flags[t]);
+        f = ld_relaxed(&flags[t]);
         if ((unsigned)(f >> 32) == epoch && ((f >> 30) & 3u) != 0u) break;
         if (++spins > (1ll << 26)) {
           spun_out = true;
@@ -279,72 +307,112 @@ PVD_DEV int look_back(uint64_t* flags, int tile, unsigned epoch, Ctrl* C, bool& 
 }
 
 // ----------------------------------------------------------------------------- discrete
+// Walkers per thread of the discrete step: a tile is TILE * W walkers, walker
+// (k, thread) of tile b is slot b*TILE*W + k*TILE + thread (coalesced per k).  Larger tiles
+// mean fewer look-back flags and shorter look-back chains.
+template <typename R, int NP, bool EXT>
+constexpr int disc_wpt() {
+  return EXT ? 4 : ((int)sizeof(R) * NP <= 36 ? 4 : ((int)sizeof(R) * NP <= 72 ? 2 : 1));
+}
+
 // MODE: INJ = injected draws; IS = importance sampling (analytic trial); EXT = energies
 // from V_ext (walkers already moved), runtime component count.
+// Tile b is block b.  Workgroups are dispatched in index order within each XCD, so the
+// predecessors a tile waits for are resident or finished; the wait is still bounded.
 template <typename R, int NP, bool INJ, bool IS, bool EXT>
 __global__ __launch_bounds__(TILE) void k_discrete(StepArgs<R> A) {
-  __shared__ int s_tile, s_prefix, s_total, s_status;
+  constexpr int W = disc_wpt<R, NP, EXT>();
+  constexpr int TW = TILE * W;
+  __shared__ int s_prefix, s_status;
   __shared__ int s_sc[TILE / PVD_WAVE];
   __shared__ double s_d[TILE / PVD_WAVE];
   __shared__ int s_i[TILE / PVD_WAVE];
   const int par = A.par;
   Ctrl* C = A.ctrl;
-  if (threadIdx.x == 0) s_tile = (int)fetch_add_u(&C->ticket[par], 1u);
-  __syncthreads();
-  const int tile = s_tile;
+  const int tile = blockIdx.x;
   const int n = C->n[par];
-  const int ntiles = (n + TILE - 1) / TILE;
+  const int ntiles = (n + TW - 1) / TW;
   if (C->status != ST_OK || tile >= ntiles) return;
+  if (tile == 0 && threadIdx.x == 0) C->ntd[par] = ntiles;
   const unsigned long long step = C->step[par];
   const unsigned epoch = C->epoch[par];
   const double vref = C->vref[par];
   double dte = A.dt;
   if (IS) dte = A.dt * ((double)C->acc[par] / (double)n);
 
-  const long long i = (long long)tile * TILE + threadIdx.x;
-  const bool valid = i < n;
-  R x[EXT ? 1 : NP];
-  double E = 0.0;
-  if (valid) {
-    if constexpr (EXT) {
-      E = A.V_ext[i];
-      if (IS) E = E + A.ke[i];
-    } else {
-      load_walker<R, NP>(A, i, x);
-      R ke = R(0);
-      if constexpr (IS) {
-        bool acc;
-        ke = is_move<R, NP, INJ>(A, i, step, x, acc);
-      } else {
-        R disp[NP];
-        draw_disp<R, NP, INJ>(A, i, step, disp);
+  R x[W][EXT ? 1 : NP];
+  int c[W];
+  double sce = 0.0;
+  int births = 0, deaths = 0, bad = 0;
 #pragma unroll
-        for (int p = 0; p < NP; ++p) x[p] = x[p] + disp[p];
+  for (int k = 0; k < W; ++k) {
+    const long long i = (long long)tile * TW + k * TILE + threadIdx.x;
+    const bool valid = i < n;
+    double E = 0.0, um = 0.0, ub = 0.0;
+    if (valid) {
+      if constexpr (EXT) {
+        E = A.V_ext[i];
+        if (IS) E = E + A.ke[i];
+      } else {
+        load_walker<R, NP>(A, i, x[k]);
+        R disp[NP];
+        draw_step<R, NP, INJ, IS>(A, i, step, disp, um, ub, A.branch != 0);
+        R ke = R(0);
+        if constexpr (IS) {
+          bool acc;
+          ke = is_move<R, NP, INJ>(A, disp, um, x[k], acc);
+        } else {
+#pragma unroll
+          for (int p = 0; p < NP; ++p) x[k][p] = x[k][p] + disp[p];
+        }
+        const R v = pes_eval<R, NP>(A.pes, x[k]);
+        E = IS ? (double)(v + ke) : (double)v;
       }
-      const R v = pes_eval<R, NP>(A.pes, x);
-      E = IS ? (double)(v + ke) : (double)v;
     }
-  }
-  // branching weight and copy count (pyvibdmc.py:392-407)
-  int c = valid ? 1 : 0;
-  int bad = 0;
-  if (valid && A.branch) {
-    const double u = draw_u<R, INJ>(A, i, step, RNG_BRANCH);
-    const double w = exp((-1. * (E - vref)) * dte);
-    if (!isfinite(w) || w > A.pop_hi + 1.0) {
-      bad = 1;
-      c = 0;
-    } else {
-      const double fl = floor(w);
-      c = (int)fl + (u < (w - fl) ? 1 : 0);
+    // branching weight and copy count (pyvibdmc.py:392-407)
+    int ck = valid ? 1 : 0;
+    if (valid && A.branch) {
+      const double u = EXT ? draw_u<R, INJ>(A, i, step, RNG_BRANCH) : ub;
+      if constexpr (sizeof(R) == 4 && !INJ) {
+        // FP32 production path: single-precision weight (the FP64 parity path below
+        // follows NumPy's double arithmetic)
+        const float wf = expf((float)((-1. * (E - vref)) * dte));
+        if (!isfinite(wf) || (double)wf > A.pop_hi + 1.0) {
+          bad = 1;
+          ck = 0;
+        } else {
+          const float fl = floorf(wf);
+          ck = (int)fl + ((float)u < (wf - fl) ? 1 : 0);
+        }
+      } else {
+        const double w = exp((-1. * (E - vref)) * dte);
+        if (!isfinite(w) || w > A.pop_hi + 1.0) {
+          bad = 1;
+          ck = 0;
+        } else {
+          const double fl = floor(w);
+          ck = (int)fl + (u < (w - fl) ? 1 : 0);
+        }
+      }
     }
+    c[k] = ck;
+    sce += (double)ck * E;
+    births += ck > 1 ? ck - 1 : 0;
+    deaths += (valid && ck == 0) ? 1 : 0;
   }
-  int total;
-  const int excl = block_excl_scan<TILE>(c, s_sc, total);
-  const double sce = block_sum<TILE>((double)c * E, s_d);
-  const int births = block_sum<TILE>(c > 1 ? c - 1 : 0, s_i);
-  const int deaths = block_sum<TILE>((valid && c == 0) ? 1 : 0, s_i);
-  const int badb = block_sum<TILE>(bad, s_i);
+  // exclusive prefix of every walker inside the tile, in slot order
+  int excl[W];
+  int total = 0;
+#pragma unroll
+  for (int k = 0; k < W; ++k) {
+    int tk;
+    excl[k] = total + block_excl_scan<TILE>(c[k], s_sc, tk);
+    total += tk;
+  }
+  sce = block_sum<TILE>(sce, s_d);
+  births = block_sum<TILE>(births, s_i);
+  deaths = block_sum<TILE>(deaths, s_i);
+  bad = block_sum<TILE>(bad, s_i);
 
   // publish the tile aggregate, then look back for the exclusive prefix
   if (threadIdx.x == 0) {
@@ -355,9 +423,9 @@ __global__ __launch_bounds__(TILE) void k_discrete(StepArgs<R> A) {
     r.count = total;
     r.births = births;
     r.deaths = deaths;
-    r.bad = badb;
+    r.bad = bad;
     A.rec[tile] = r;
-    st_release(&A.flags[tile], pack_flag(epoch, tile == 0 ? FL_INC : FL_AGG, (unsigned)total));
+    st_relaxed(&A.flags[tile], pack_flag(epoch, tile == 0 ? FL_INC : FL_AGG, (unsigned)total));
   }
   if (threadIdx.x < PVD_WAVE) {
     int prefix = 0;
@@ -367,7 +435,7 @@ __global__ __launch_bounds__(TILE) void k_discrete(StepArgs<R> A) {
       if (spun) {
         st_release_i(&C->status, ST_SPIN);
       } else if (tile > 0) {
-        st_release(&A.flags[tile], pack_flag(epoch, FL_INC, (unsigned)(prefix + total)));
+        st_relaxed(&A.flags[tile], pack_flag(epoch, FL_INC, (unsigned)(prefix + total)));
       }
       s_prefix = prefix;
       s_status = spun ? 1 : 0;
@@ -375,29 +443,45 @@ __global__ __launch_bounds__(TILE) void k_discrete(StepArgs<R> A) {
   }
   __syncthreads();
   if (s_status) return;
-  const long long base = (long long)s_prefix + excl;
 
   // survivors to their np.repeat positions
-  if (valid && c > 0) {
-    for (int k = 0; k < c; ++k) {
-      const long long dst = base + k;
-      if (dst >= A.cap) break;
-      if constexpr (EXT) {
-        for (int p = 0; p < A.np; ++p) A.cout[p * A.cap + dst] = A.cin[p * A.cap + i];
-      } else {
 #pragma unroll
-        for (int p = 0; p < NP; ++p) A.cout[p * A.cap + dst] = x[p];
+  for (int k = 0; k < W; ++k) {
+    const long long i = (long long)tile * TW + k * TILE + threadIdx.x;
+    if (i < n && c[k] > 0) {
+      const long long base = (long long)s_prefix + excl[k];
+      for (int q = 0; q < c[k]; ++q) {
+        const long long dst = base + q;
+        if (dst >= A.cap) break;
+        if constexpr (EXT) {
+          for (int p = 0; p < A.np; ++p) A.cout[p * A.cap + dst] = A.cin[p * A.cap + i];
+        } else {
+#pragma unroll
+          for (int p = 0; p < NP; ++p) A.cout[p * A.cap + dst] = x[k][p];
+        }
+        if (A.who_out) A.who_out[dst] = A.who_in[i];
       }
-      if (A.who_out) A.who_out[dst] = A.who_in[i];
     }
   }
+}
 
-  // the last tile in scan order finalises the step
-  if (tile != ntiles - 1) return;
-  fence_acquire();
+// Finalise a discrete step in one workgroup: fixed-order reduction of the per-tile
+// partials, the population guards (pyvibdmc.py:397-400, 409-413), calc_vref
+// (pyvibdmc.py:655-661) and the per-step record (pyvibdmc.py:674-682).
+constexpr int FN_NT = 1024;
+template <typename R, bool IS>
+__global__ __launch_bounds__(FN_NT) void k_finalize_discrete(StepArgs<R> A) {
+  __shared__ double s_d[FN_NT / PVD_WAVE];
+  __shared__ int s_i[FN_NT / PVD_WAVE];
+  const int par = A.par;
+  Ctrl* C = A.ctrl;
+  if (C->status != ST_OK) return;
+  const int n = C->n[par];
+  const int ntiles = C->ntd[par];
+  const unsigned long long step = C->step[par];
   double s0 = 0.0;
   int cnt = 0, nb = 0, nd = 0, nbad = 0;
-  for (int t = threadIdx.x; t < ntiles; t += TILE) {
+  for (int t = threadIdx.x; t < ntiles; t += FN_NT) {
     const TileRec r = A.rec[t];
     s0 += r.s0;
     cnt += r.count;
@@ -405,42 +489,56 @@ __global__ __launch_bounds__(TILE) void k_discrete(StepArgs<R> A) {
     nd += r.deaths;
     nbad += r.bad;
   }
-  s0 = block_sum<TILE>(s0, s_d);
-  cnt = block_sum<TILE>(cnt, s_i);
-  nb = block_sum<TILE>(nb, s_i);
-  nd = block_sum<TILE>(nd, s_i);
-  nbad = block_sum<TILE>(nbad, s_i);
-  if (threadIdx.x == 0) {
-    const int np1 = par ^ 1;
-    int st = ST_OK;
-    if (nbad) st = ST_WEIGHT;
-    else if ((double)cnt < A.pop_lo || (double)cnt > A.pop_hi) st = ST_POP;
-    else if (cnt > A.cap) st = ST_CAPACITY;
-    if (st != ST_OK) {
-      C->status = st;
-      C->status_step = (long long)step;
-    } else {
-      // calc_vref, discrete (pyvibdmc.py:655-661)
-      const double vbar = s0 / (double)cnt;
-      const double corr = (double)(cnt - (int)A.n0) / A.n0;
-      const double vnew = vbar - (A.alpha * corr);
-      C->n[np1] = cnt;
-      C->vref[np1] = vnew;
-      C->step[np1] = step + 1;
-      C->epoch[np1] = epoch + 1;
-      C->ticket[np1] = 0u;
-      if ((long long)step < A.tmax) {
-        A.tr_vref[step] = vnew;
-        A.tr_pop[step] = (double)cnt;
-        A.tr_births[step] = nb;
-        A.tr_deaths[step] = nd;
-        if (IS) {
-          C->eff_t = (step == 0 ? 0.0 : C->eff_t) + dte;
-          A.tr_dteff[step] = C->eff_t;
-        }
-      }
+  s0 = block_sum<FN_NT>(s0, s_d);
+  cnt = block_sum<FN_NT>(cnt, s_i);
+  nb = block_sum<FN_NT>(nb, s_i);
+  nd = block_sum<FN_NT>(nd, s_i);
+  nbad = block_sum<FN_NT>(nbad, s_i);
+  if (threadIdx.x != 0) return;
+  const int np1 = par ^ 1;
+  int st = ST_OK;
+  if (nbad) st = ST_WEIGHT;
+  else if ((double)cnt < A.pop_lo || (double)cnt > A.pop_hi) st = ST_POP;
+  else if (cnt > A.cap) st = ST_CAPACITY;
+  if (st != ST_OK) {
+    C->status = st;
+    C->status_step = (long long)step;
+    return;
+  }
+  const double vbar = s0 / (double)cnt;
+  const double corr = (double)(cnt - (int)A.n0) / A.n0;
+  const double vnew = vbar - (A.alpha * corr);
+  C->n[np1] = cnt;
+  C->vref[np1] = vnew;
+  C->step[np1] = step + 1;
+  C->epoch[np1] = C->epoch[par] + 1;
+  C->ticket[np1] = 0u;
+  if ((long long)step < A.tmax) {
+    A.tr_vref[step] = vnew;
+    A.tr_pop[step] = (double)cnt;
+    A.tr_births[step] = nb;
+    A.tr_deaths[step] = nd;
+    if (IS) {
+      const double dte = A.dt * ((double)C->acc[par] / (double)n);
+      C->eff_t = (step == 0 ? 0.0 : C->eff_t) + dte;
+      A.tr_dteff[step] = C->eff_t;
     }
   }
+}
+
+// Total of the per-tile accepted-move counts of the coming step (IS).
+template <typename R>
+__global__ __launch_bounds__(FN_NT) void k_acc_finalize(StepArgs<R> A) {
+  __shared__ int s_i[FN_NT / PVD_WAVE];
+  const int par = A.par;
+  Ctrl* C = A.ctrl;
+  if (C->status != ST_OK) return;
+  const int n = C->n[par];
+  const int ntiles = (n + TILE - 1) / TILE;
+  int tot = 0;
+  for (int t = threadIdx.x; t < ntiles; t += FN_NT) tot += A.rec[t].births;
+  tot = block_sum<FN_NT>(tot, s_i);
+  if (threadIdx.x == 0) C->acc[par] = tot;
 }
 
 // -------------------------------------------------------------- acceptance pre-pass (IS)
@@ -449,7 +547,6 @@ __global__ __launch_bounds__(TILE) void k_discrete(StepArgs<R> A) {
 template <typename R, int NP, bool INJ>
 __global__ __launch_bounds__(TILE) void k_accept(StepArgs<R> A) {
   __shared__ int s_i[TILE / PVD_WAVE];
-  __shared__ unsigned s_last;
   const int par = A.par;
   Ctrl* C = A.ctrl;
   const int n = C->n[par];
@@ -460,28 +557,16 @@ __global__ __launch_bounds__(TILE) void k_accept(StepArgs<R> A) {
   const long long i = (long long)tile * TILE + threadIdx.x;
   int a = 0;
   if (i < n) {
-    R x[NP];
+    R x[NP], disp[NP];
+    double um, ub;
     load_walker<R, NP>(A, i, x);
+    draw_step<R, NP, INJ, true>(A, i, step, disp, um, ub, false);
     bool acc;
-    is_move<R, NP, INJ>(A, i, step, x, acc);
+    is_move<R, NP, INJ>(A, disp, um, x, acc);
     a = acc ? 1 : 0;
   }
   a = block_sum<TILE>(a, s_i);
-  if (threadIdx.x == 0) {
-    A.rec[tile].count = a;
-    fence_release();
-    s_last = fetch_add_u(&C->done[par], 1u);
-  }
-  __syncthreads();
-  if (s_last != (unsigned)(ntiles - 1)) return;
-  fence_acquire();
-  int tot = 0;
-  for (int t = threadIdx.x; t < ntiles; t += TILE) tot += A.rec[t].count;
-  tot = block_sum<TILE>(tot, s_i);
-  if (threadIdx.x == 0) {
-    C->acc[par] = tot;
-    C->done[par] = 0u;
-  }
+  if (threadIdx.x == 0) A.rec[tile].births = a;
 }
 
 // ------------------------------------------------------------------ split path: the move
@@ -491,7 +576,6 @@ __global__ __launch_bounds__(TILE) void k_accept(StepArgs<R> A) {
 template <typename R, int NP, bool INJ, bool IS>
 __global__ __launch_bounds__(TILE) void k_move(StepArgs<R> A) {
   __shared__ int s_i[TILE / PVD_WAVE];
-  __shared__ unsigned s_last;
   const int par = A.par;
   Ctrl* C = A.ctrl;
   const int n = C->n[par];
@@ -503,10 +587,12 @@ __global__ __launch_bounds__(TILE) void k_move(StepArgs<R> A) {
   int a = 0;
   if (i < n) {
     if constexpr (IS) {
-      R x[NP];
+      R x[NP], disp[NP];
+      double um, ub;
       load_walker<R, NP>(A, i, x);
+      draw_step<R, NP, INJ, true>(A, i, step, disp, um, ub, false);
       bool acc;
-      const R ke = is_move<R, NP, INJ>(A, i, step, x, acc);
+      const R ke = is_move<R, NP, INJ>(A, disp, um, x, acc);
       a = acc ? 1 : 0;
       A.ke[i] = (double)ke;
 #pragma unroll
@@ -543,21 +629,7 @@ __global__ __launch_bounds__(TILE) void k_move(StepArgs<R> A) {
   }
   if constexpr (IS) {
     a = block_sum<TILE>(a, s_i);
-    if (threadIdx.x == 0) {
-      A.rec[tile].count = a;
-      fence_release();
-      s_last = fetch_add_u(&C->done[par], 1u);
-    }
-    __syncthreads();
-    if (s_last != (unsigned)(ntiles - 1)) return;
-    fence_acquire();
-    int tot = 0;
-    for (int t = threadIdx.x; t < ntiles; t += TILE) tot += A.rec[t].count;
-    tot = block_sum<TILE>(tot, s_i);
-    if (threadIdx.x == 0) {
-      C->acc[par] = tot;
-      C->done[par] = 0u;
-    }
+    if (threadIdx.x == 0) A.rec[tile].births = a;
   }
 }
 
@@ -568,7 +640,6 @@ template <typename R, int NP, bool INJ, bool IS, bool EXT>
 __global__ __launch_bounds__(TILE) void k_cont(StepArgs<R> A) {
   __shared__ double s_d[TILE / PVD_WAVE];
   __shared__ int s_i[TILE / PVD_WAVE];
-  __shared__ unsigned s_last;
   const int par = A.par;
   Ctrl* C = A.ctrl;
   const int n = C->n[par];
@@ -588,15 +659,15 @@ __global__ __launch_bounds__(TILE) void k_cont(StepArgs<R> A) {
       E = A.V_ext[i];
       if (IS) E = E + A.ke[i];
     } else {
-      R x[NP];
+      R x[NP], disp[NP];
+      double um, ub;
       load_walker<R, NP>(A, i, x);
+      draw_step<R, NP, INJ, IS>(A, i, step, disp, um, ub, false);
       R ke = R(0);
       if constexpr (IS) {
         bool acc;
-        ke = is_move<R, NP, INJ>(A, i, step, x, acc);
+        ke = is_move<R, NP, INJ>(A, disp, um, x, acc);
       } else {
-        R disp[NP];
-        draw_disp<R, NP, INJ>(A, i, step, disp);
 #pragma unroll
         for (int p = 0; p < NP; ++p) x[p] = x[p] + disp[p];
       }
@@ -626,45 +697,6 @@ __global__ __launch_bounds__(TILE) void k_cont(StepArgs<R> A) {
     r.deaths = 0;
     r.bad = 0;
     A.rec[tile] = r;
-    fence_release();
-    s_last = fetch_add_u(&C->done[par], 1u);
-  }
-  __syncthreads();
-  if (s_last != (unsigned)(ntiles - 1)) return;
-  fence_acquire();
-  double a0 = 0.0, a1 = 0.0;
-  int k = 0;
-  for (int t = threadIdx.x; t < ntiles; t += TILE) {
-    a0 += A.rec[t].s0;
-    a1 += A.rec[t].s1;
-    k += A.rec[t].count;
-  }
-  a0 = block_sum<TILE>(a0, s_d);
-  a1 = block_sum<TILE>(a1, s_d);
-  k = block_sum<TILE>(k, s_i);
-  if (threadIdx.x == 0) {
-    const int np1 = par ^ 1;
-    C->done[par] = 0u;
-    C->kills[par] = k;
-    C->sumw[par] = a1;
-    C->sumwe[par] = a0;
-    C->n[np1] = n;
-    C->step[np1] = step + 1;
-    C->epoch[np1] = C->epoch[par] + 1;
-    if (IS && (long long)step < A.tmax) {
-      C->eff_t = (step == 0 ? 0.0 : C->eff_t) + dte;
-      A.tr_dteff[step] = C->eff_t;
-    }
-    if (k == 0) {
-      // calc_vref, continuous (pyvibdmc.py:659-661)
-      const double vnew = a0 / a1 - (A.alpha * ((a1 - A.n0) / A.n0));
-      C->vref[np1] = vnew;
-      if ((long long)step < A.tmax) {
-        A.tr_vref[step] = vnew;
-        A.tr_pop[step] = a1;
-        A.tr_births[step] = 0;
-      }
-    }
   }
 }
 
@@ -713,15 +745,55 @@ __global__ __launch_bounds__(RP_NT) void k_cont_replay(StepArgs<R> A) {
   __shared__ int s_cnt, s_ncur, s_npend, s_nfin, s_fail, s_next;
   __shared__ double s_d[RP_NT / PVD_WAVE];
   __shared__ int s_i[RP_NT / PVD_WAVE];
+  __shared__ int s_K;
   const int par = A.par;
   Ctrl* C = A.ctrl;
   const int np1 = par ^ 1;
-  const int K = C->kills[par];
-  if (C->status != ST_OK || K == 0) return;
+  if (C->status != ST_OK) return;
   const int n = C->n[par];
   const int ntiles = (n + TILE - 1) / TILE;
   const unsigned long long step = C->step[par];
   const unsigned long long ep = (unsigned long long)C->epoch[par];
+  // 0. fixed-order reduction of the per-tile sums; without splits this finishes the step
+  //    (calc_vref, continuous: pyvibdmc.py:659-661)
+  {
+    double a0 = 0.0, a1 = 0.0;
+    int k = 0;
+    for (int t = threadIdx.x; t < ntiles; t += RP_NT) {
+      a0 += A.rec[t].s0;
+      a1 += A.rec[t].s1;
+      k += A.rec[t].count;
+    }
+    a0 = block_sum<RP_NT>(a0, s_d);
+    a1 = block_sum<RP_NT>(a1, s_d);
+    k = block_sum<RP_NT>(k, s_i);
+    if (threadIdx.x == 0) {
+      s_K = k;
+      C->kills[par] = k;
+      C->sumw[par] = a1;
+      C->sumwe[par] = a0;
+      C->n[np1] = n;
+      C->step[np1] = step + 1;
+      C->epoch[np1] = C->epoch[par] + 1;
+      if (A.is_ && (long long)step < A.tmax) {
+        const double dte = A.dt * ((double)C->acc[par] / (double)n);
+        C->eff_t = (step == 0 ? 0.0 : C->eff_t) + dte;
+        A.tr_dteff[step] = C->eff_t;
+      }
+      if (k == 0) {
+        const double vnew = a0 / a1 - (A.alpha * ((a1 - A.n0) / A.n0));
+        C->vref[np1] = vnew;
+        if ((long long)step < A.tmax) {
+          A.tr_vref[step] = vnew;
+          A.tr_pop[step] = a1;
+          A.tr_births[step] = 0;
+        }
+      }
+    }
+    __syncthreads();
+  }
+  const int K = s_K;
+  if (K == 0) return;
   const unsigned long long tagbase = ep << 32;
   const int tid = threadIdx.x;
   int* kl = A.klist;

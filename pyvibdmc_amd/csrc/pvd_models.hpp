@@ -140,7 +140,9 @@ PVD_DEV R pes_eval(const PesParams& P, const R* x) {
     }
     case PES_MORSE_BEND:
       if constexpr (NP >= 9) {
-        // gather (h1, h2, o) with compile-time selects so the walker stays in registers
+        if (P.idx[0] == 0 && P.idx[1] == 1 && P.idx[2] == 2)  // [H, H, O], the usual order
+          return morse_bend_v<R>(x, 0, 3, 6, R(P.p[0]), R(P.p[1]), R(P.p[2]), R(0.5 * P.p[3]), R(P.p[4]));
+        // other orders: gather (h1, h2, o) with selects so the walker stays in registers
         R g[9];
         gather3<R, NP>(x, P.idx, g);
         return morse_bend_v<R>(g, 0, 3, 6, R(P.p[0]), R(P.p[1]), R(P.p[2]), R(0.5 * P.p[3]), R(P.p[4]));
@@ -273,6 +275,98 @@ struct TrialOut {
   R d2[NP];
 };
 
+// Gradient / diagonal-Hessian contributions of one Gaussian factor in a bond length
+// r = |x_i - x_j| (component offsets i, j): ln psi -= al (r - r0)^2 / 2.
+template <typename R, int NP>
+PVD_DEV void bond_term(const R* x, int i, int j, R al, R r0, R* g, R* h, R& q2) {
+  R u[3];
+#pragma unroll
+  for (int c = 0; c < 3; ++c) {
+    R xi = R(0), xj = R(0);
+#pragma unroll
+    for (int p = 0; p < NP; ++p) {
+      if (p == i + c) xi = x[p];
+      if (p == j + c) xj = x[p];
+    }
+    u[c] = xi - xj;
+  }
+  const R r = norm3(u[0], u[1], u[2]);
+  const R dq = r - r0;
+  q2 += al * (dq * dq);
+#pragma unroll
+  for (int c = 0; c < 3; ++c) {
+    const R e = u[c] / r;
+    const R d2 = (R(1) - e * e) / r;
+    const R t1 = -al * dq * e;
+    const R t2 = -al * (e * e + dq * d2);
+#pragma unroll
+    for (int p = 0; p < NP; ++p) {
+      if (p == i + c) {
+        g[p] += t1;
+        h[p] += t2;
+      }
+      if (p == j + c) {
+        g[p] -= t1;
+        h[p] += t2;
+      }
+    }
+  }
+}
+
+// One Gaussian factor in the bond angle i-j-k (vertex j), through c = cos(theta):
+// dtheta = -dc/s, d2theta = -d2c/s - c dc^2/s^3, s = sin(theta).
+template <typename R, int NP>
+PVD_DEV void angle_term(const R* x, int i, int j, int kk, R al, R th0, R* g, R* h, R& q2) {
+  R u[3], v[3];
+#pragma unroll
+  for (int c = 0; c < 3; ++c) {
+    R xi = R(0), xj = R(0), xk = R(0);
+#pragma unroll
+    for (int p = 0; p < NP; ++p) {
+      if (p == i + c) xi = x[p];
+      if (p == j + c) xj = x[p];
+      if (p == kk + c) xk = x[p];
+    }
+    u[c] = xi - xj;
+    v[c] = xk - xj;
+  }
+  const R ru = norm3(u[0], u[1], u[2]), rv = norm3(v[0], v[1], v[2]);
+  R uh[3], vh[3];
+#pragma unroll
+  for (int c = 0; c < 3; ++c) {
+    uh[c] = u[c] / ru;
+    vh[c] = v[c] / rv;
+  }
+  const R cth = clamp1(uh[0] * vh[0] + uh[1] * vh[1] + uh[2] * vh[2]);
+  const R s = pvd_sqrt<R>(R(1) - cth * cth);
+  const R dq = pvd_acos<R>(cth) - th0;
+  q2 += al * (dq * dq);
+#pragma unroll
+  for (int c = 0; c < 3; ++c) {
+    const R dcu = (vh[c] - cth * uh[c]) / ru;
+    const R dcv = (uh[c] - cth * vh[c]) / rv;
+    const R duu = (R(-2) * uh[c] * vh[c] - cth + R(3) * cth * uh[c] * uh[c]) / (ru * ru);
+    const R dvv = (R(-2) * uh[c] * vh[c] - cth + R(3) * cth * vh[c] * vh[c]) / (rv * rv);
+    const R duv = (R(1) - uh[c] * uh[c] - vh[c] * vh[c] + cth * uh[c] * vh[c]) / (ru * rv);
+    const R dcs[3] = {dcu, dcv, -(dcu + dcv)};
+    const R d2cs[3] = {duu, dvv, duu + R(2) * duv + dvv};
+    const int at[3] = {i + c, kk + c, j + c};
+#pragma unroll
+    for (int z = 0; z < 3; ++z) {
+      const R dth = -dcs[z] / s;
+      const R d2th = -d2cs[z] / s - cth * dcs[z] * dcs[z] / (s * s * s);
+      const R t1 = -al * dq * dth;
+      const R t2 = -al * (dth * dth + dq * d2th);
+#pragma unroll
+      for (int p = 0; p < NP; ++p)
+        if (p == at[z]) {
+          g[p] += t1;
+          h[p] += t2;
+        }
+    }
+  }
+}
+
 template <typename R, int NP>
 PVD_DEV void trial_eval(const TrialParams& T, const R* x, TrialOut<R, NP>& o) {
   if (T.kind == TRIAL_HARMONIC_1D) {
@@ -303,78 +397,17 @@ PVD_DEV void trial_eval(const TrialParams& T, const R* x, TrialOut<R, NP>& o) {
   }
   R q2 = R(0);
   if constexpr (NP % 3 == 0) {
-    for (int k = 0; k < T.nb; ++k) {
-      const int i = 3 * T.bi[k], j = 3 * T.bj[k];
-      const R al = R(T.balpha[k]);
-      R u[3];
-#pragma unroll
-      for (int c = 0; c < 3; ++c) u[c] = x[i + c] - x[j + c];
-      const R r = norm3(u[0], u[1], u[2]);
-      const R dq = r - R(T.br0[k]);
-      q2 += al * (dq * dq);
-#pragma unroll
-      for (int c = 0; c < 3; ++c) {
-        const R e = u[c] / r;
-        const R d2 = (R(1) - e * e) / r;
-        const R t1 = -al * dq * e;
-        const R t2 = -al * (e * e + dq * d2);
-#pragma unroll
-        for (int p = 0; p < NP; ++p) {
-          if (p == i + c) {
-            g[p] += t1;
-            h[p] += t2;
-          }
-          if (p == j + c) {
-            g[p] -= t1;
-            h[p] += t2;
-          }
-        }
-      }
-    }
-    for (int k = 0; k < T.na; ++k) {
-      const int i = 3 * T.ai[k], j = 3 * T.aj[k], kk = 3 * T.ak[k];
-      const R al = R(T.aalpha[k]);
-      R u[3], v[3];
-#pragma unroll
-      for (int c = 0; c < 3; ++c) {
-        u[c] = x[i + c] - x[j + c];
-        v[c] = x[kk + c] - x[j + c];
-      }
-      const R ru = norm3(u[0], u[1], u[2]), rv = norm3(v[0], v[1], v[2]);
-      R uh[3], vh[3];
-#pragma unroll
-      for (int c = 0; c < 3; ++c) {
-        uh[c] = u[c] / ru;
-        vh[c] = v[c] / rv;
-      }
-      const R cth = clamp1(uh[0] * vh[0] + uh[1] * vh[1] + uh[2] * vh[2]);
-      const R s = pvd_sqrt<R>(R(1) - cth * cth);
-      const R dq = pvd_acos<R>(cth) - R(T.ath0[k]);
-      q2 += al * (dq * dq);
-#pragma unroll
-      for (int c = 0; c < 3; ++c) {
-        const R dcu = (vh[c] - cth * uh[c]) / ru;
-        const R dcv = (uh[c] - cth * vh[c]) / rv;
-        const R duu = (R(-2) * uh[c] * vh[c] - cth + R(3) * cth * uh[c] * uh[c]) / (ru * ru);
-        const R dvv = (R(-2) * uh[c] * vh[c] - cth + R(3) * cth * vh[c] * vh[c]) / (rv * rv);
-        const R duv = (R(1) - uh[c] * uh[c] - vh[c] * vh[c] + cth * uh[c] * vh[c]) / (ru * rv);
-        const R dcs[3] = {dcu, dcv, -(dcu + dcv)};
-        const R d2cs[3] = {duu, dvv, duu + R(2) * duv + dvv};
-        const int at[3] = {i + c, kk + c, j + c};
-#pragma unroll
-        for (int z = 0; z < 3; ++z) {
-          const R dth = -dcs[z] / s;
-          const R d2th = -d2cs[z] / s - cth * dcs[z] * dcs[z] / (s * s * s);
-          const R t1 = -al * dq * dth;
-          const R t2 = -al * (dth * dth + dq * d2th);
-#pragma unroll
-          for (int p = 0; p < NP; ++p)
-            if (p == at[z]) {
-              g[p] += t1;
-              h[p] += t2;
-            }
-        }
-      }
+    if (NP == 9 && T.nb == 2 && T.na == 1 && T.bi[0] == 0 && T.bj[0] == 2 && T.bi[1] == 1 && T.bj[1] == 2 &&
+        T.ai[0] == 0 && T.aj[0] == 2 && T.ak[0] == 1) {
+      // [H, H, O] water pattern with constant indices
+      bond_term<R, NP>(x, 0, 6, R(T.balpha[0]), R(T.br0[0]), g, h, q2);
+      bond_term<R, NP>(x, 3, 6, R(T.balpha[1]), R(T.br0[1]), g, h, q2);
+      angle_term<R, NP>(x, 0, 6, 3, R(T.aalpha[0]), R(T.ath0[0]), g, h, q2);
+    } else {
+      for (int k = 0; k < T.nb; ++k)
+        bond_term<R, NP>(x, 3 * T.bi[k], 3 * T.bj[k], R(T.balpha[k]), R(T.br0[k]), g, h, q2);
+      for (int k = 0; k < T.na; ++k)
+        angle_term<R, NP>(x, 3 * T.ai[k], 3 * T.aj[k], 3 * T.ak[k], R(T.aalpha[k]), R(T.ath0[k]), g, h, q2);
     }
   }
   o.lnpsi = R(T.c0) - R(0.5) * q2;

@@ -149,3 +149,31 @@ def test_restart_from_checkpoint_continues_the_same_draws(tmp_path):
     c.propagate()
     np.testing.assert_array_equal(b._pop_vs_tau, c._pop_vs_tau)
     np.testing.assert_allclose(b._vref_vs_tau, c._vref_vs_tau, rtol=0, atol=1e-12)
+
+
+@pytest.mark.parametrize('mode', ['discrete', 'continuous', 'is'])
+def test_graph_replay_matches_eager_launches(monkeypatch, mode):
+    """pvd_sim_run replays a 64-step HIP graph; 300 steps (4 replays + remainder) must give
+    the same populations, vref and walkers bit for bit as one launch at a time."""
+    import pyvibdmc_amd as pv
+    runs = []
+    for graphs in ('1', '0'):
+        monkeypatch.setenv('PVD_GRAPHS', graphs)
+        kw = {}
+        if mode == 'continuous':
+            kw = dict(weighting='continuous', cont_wt_thresh=0.05)
+        if mode == 'is':
+            kw = dict(imp_samp=pv.water_trial())
+        sim = pv.DMC_Sim(num_walkers=20000, num_timesteps=300, atoms=['H', 'H', 'O'],
+                         masses=[1837.15, 1837.15, 29156.95], delta_t=5, potential=pv.MorseBendTriatomic(),
+                         start_structures=water(20000, 12, 0.05), precision='fp32', seed=7, write_files=False,
+                         equil_steps=10 ** 9, wfn_every=10 ** 9, chkpt_every=10 ** 9, **kw)
+        sim.propagate()
+        c, w = sim._get_walkers()
+        runs.append((sim._pop_vs_tau.copy(), sim._vref_vs_tau.copy(), c, w))
+    (p1, v1, c1, w1), (p2, v2, c2, w2) = runs
+    np.testing.assert_array_equal(p1, p2)
+    np.testing.assert_array_equal(v1, v2)
+    np.testing.assert_array_equal(c1, c2)
+    if w1 is not None:
+        np.testing.assert_array_equal(w1, w2)
