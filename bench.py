@@ -137,7 +137,7 @@ def main():
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU: the DMC walkers of this workload live on a single MI355X "
-              f"(--gpus {args.gpus} is not supported; multi-GPU sharding is described in DESIGN.md).",
+              f"(--gpus {args.gpus} is not supported: this build runs on one GPU only).",
               file=sys.stderr)
         sys.exit(2)
     import torch

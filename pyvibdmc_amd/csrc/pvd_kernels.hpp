@@ -723,8 +723,13 @@ constexpr int RP_KCAP = 4096;  // kills per step
 constexpr int RP_PCAP = RP_KCAP * 2;  // pending pieces per buffer
 constexpr int RP_FCAP = RP_KCAP * 4;  // final pieces
 
+// total order (weight descending, then slot, birth, source): the sorted interval does not
+// depend on the order in which its members were gathered
 PVD_DEV bool piece_before(const Piece& a, const Piece& b) {
-  return a.v > b.v || (a.v == b.v && a.slot < b.slot);
+  if (a.v != b.v) return a.v > b.v;
+  if (a.slot != b.slot) return a.slot < b.slot;
+  if (a.birth != b.birth) return a.birth < b.birth;
+  return a.src < b.src;
 }
 
 // kill index of `slot` in this launch (tagged with the launch epoch), or INT_MAX
@@ -891,10 +896,27 @@ __global__ __launch_bounds__(RP_NT) void k_cont_replay(StepArgs<R> A) {
     __syncthreads();
     if (tid == 0) s_npend = 0;
     __syncthreads();
-    for (int q = tid; q < npend; q += RP_NT) {
-      const Piece pc = pin_list[q];
-      if (pc.v >= L && pc.v < H) cur[atomicAdd(&s_ncur, 1)] = pc;
-      else pout_list[atomicAdd(&s_npend, 1)] = pc;
+    // in-range pieces join the interval (its order is fixed by the sort below); the
+    // others are carried over in their input order (scan offsets, deterministic)
+    {
+      int pbase = 0;
+      for (int q0 = 0; q0 < npend; q0 += RP_NT) {
+        const int q = q0 + tid;
+        Piece pc;
+        bool in = false, keep = false;
+        if (q < npend) {
+          pc = pin_list[q];
+          in = pc.v >= L && pc.v < H;
+          keep = !in;
+        }
+        if (in) cur[atomicAdd(&s_ncur, 1)] = pc;
+        int tot;
+        const int ex = block_excl_scan<RP_NT>(keep ? 1 : 0, s_i, tot);
+        if (keep) pout_list[pbase + ex] = pc;
+        pbase += tot;
+      }
+      if (tid == 0) s_npend = pbase;
+      __syncthreads();
     }
     for (int t = wv; t < ntiles; t += RP_NT / PVD_WAVE) {
       if (A.rec[t].mx < L) continue;
@@ -947,29 +969,47 @@ __global__ __launch_bounds__(RP_NT) void k_cont_replay(StepArgs<R> A) {
       }
     }
     if (!degenerate) {
-      // members [0, m) are chosen in order and split; the rest are final
+      // members [0, m) are chosen in order and split (children at fixed positions); the
+      // rest are final (appended at scan offsets)
       const int m = min(ncur, K - next);
-      for (int q = tid; q < ncur; q += RP_NT) {
-        const Piece pc = cur[q];
-        if (q < m) {
-          Piece a = pc, b = pc;
-          a.v = pc.v * 0.5;
-          a.birth = next + q;
-          b.v = a.v;
-          b.birth = next + q;
-          b.slot = kl[next + q];
-          const int o = atomicAdd(&s_npend, 2);
-          if (o + 2 <= RP_PCAP) {
-            pout_list[o] = a;
-            pout_list[o + 1] = b;
+      const int pb = s_npend, fb = s_nfin;
+      int fbase = fb;
+      for (int q0 = 0; q0 < ncur; q0 += RP_NT) {
+        const int q = q0 + tid;
+        bool fin_here = false;
+        Piece pc;
+        if (q < ncur) {
+          pc = cur[q];
+          if (q < m) {
+            Piece a = pc, b = pc;
+            a.v = pc.v * 0.5;
+            a.birth = next + q;
+            b.v = a.v;
+            b.birth = next + q;
+            b.slot = kl[next + q];
+            const int o = pb + 2 * q;
+            if (o + 2 <= RP_PCAP) {
+              pout_list[o] = a;
+              pout_list[o + 1] = b;
+            } else {
+              s_fail = 1;
+            }
           } else {
-            s_fail = 1;
+            fin_here = pc.birth >= 0;
           }
-        } else if (pc.birth >= 0) {
-          const int o = atomicAdd(&s_nfin, 1);
-          if (o < RP_FCAP) A.fin[o] = pc;
+        }
+        int tot;
+        const int ex = block_excl_scan<RP_NT>(fin_here ? 1 : 0, s_i, tot);
+        if (fin_here) {
+          if (fbase + ex < RP_FCAP) A.fin[fbase + ex] = pc;
           else s_fail = 1;
         }
+        fbase += tot;
+      }
+      __syncthreads();
+      if (tid == 0) {
+        s_npend = pb + 2 * m;
+        s_nfin = fbase;
       }
       next += m;
     } else {
@@ -1023,12 +1063,14 @@ __global__ __launch_bounds__(RP_NT) void k_cont_replay(StepArgs<R> A) {
   // every still-pending piece is final
   {
     const int npend = s_npend;
+    const int fb = s_nfin;
     const Piece* pl = A.pend + pbuf * RP_PCAP;
     for (int q = tid; q < npend; q += RP_NT) {
-      const int o = atomicAdd(&s_nfin, 1);
-      if (o < RP_FCAP) A.fin[o] = pl[q];
+      if (fb + q < RP_FCAP) A.fin[fb + q] = pl[q];
       else s_fail = 1;
     }
+    __syncthreads();
+    if (tid == 0) s_nfin = fb + npend;
     __syncthreads();
     if (s_fail) {
       if (tid == 0) {
