@@ -5,7 +5,7 @@ Workload (the metric's headline case): plain DMC on the triatomic Morse+bend PES
 theta0=104.5 deg), 1M walkers started at the equilibrium geometry + N(0, 0.05 bohr),
 dt = 5 a.u., discrete birth/death, FP32 coordinates and per-walker PES with FP64 weights
 and reductions, Philox draws.  A step is one fused kernel launch (move, PES, weights,
-branching, compaction, vref); value = walker-steps/s = sum over the K timed steps of the
+branching, compaction) and a one-block vref kernel; value = walker-steps/s = sum over the K timed steps of the
 population entering each step, divided by the wall time between two device syncs.
 
 Also reported: the same case with importance sampling (Gaussian-product psi_T, drift and
@@ -74,15 +74,19 @@ def run_timed(sim, warmup, steps, events_steps=0):
     ws = float(np.sum(entering))
     kern_ms = None
     if events_steps:
-        evs = [(torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)) for _ in range(events_steps)]
-        for a, b in evs:
-            a.record()
-            N.check(lib.pvd_sim_run(sim._h, 1, st))
-            b.record()
-        torch.cuda.synchronize()
+        # HIP event pairs around every kernel of each step, recorded by the library on the
+        # launch stream; kern_ms[k] = mean duration of the step's k-th kernel
+        import ctypes
+        avg = (ctypes.c_float * 8)()
+        nk = ctypes.c_int32(0)
+        N.check(lib.pvd_sim_run_timed(sim._h, events_steps, st, avg, ctypes.byref(nk)))
         sim._raise_if_failed()
-        kern_ms = float(np.mean([a.elapsed_time(b) for a, b in evs]))
-    return ws, wall, float(np.mean(entering)), kern_ms
+        kern_ms = [float(avg[k]) for k in range(nk.value)]
+        sim._fetch_traj(0, warmup + steps + events_steps)
+        ev_n = float(np.mean(sim._pop_vs_tau[warmup + steps - 1:warmup + steps + events_steps - 1]))
+    else:
+        ev_n = None
+    return ws, wall, float(np.mean(entering)), kern_ms, ev_n
 
 
 def cpu_baseline(n):
@@ -151,22 +155,26 @@ def main():
 
     log(f'[bench] plain {args.precision} {n} walkers: warmup {W}, timed {K}')
     sim = make_sim(n, total, precision=args.precision)
-    ws, wall, mean_n, kern_ms = run_timed(sim, W, K, args.event_steps)
+    ws, wall, mean_n, kern_ms, ev_n = run_timed(sim, W, K, args.event_steps)
     value = ws / wall
     ms_per_step = wall * 1e3 / K
     bpw = BYTES_PER_WALKER_STEP[('plain', args.precision, 9)]
     roof = None
     if kern_ms:
-        bytes_per_launch = bpw * mean_n
-        achieved = bytes_per_launch / (kern_ms * 1e-3) / 1e9
+        # plain discrete step = [k_discrete, k_finalize_discrete]; the first dominates
+        k_ms = kern_ms[0]
+        bytes_per_launch = bpw * ev_n
+        achieved = bytes_per_launch / (k_ms * 1e-3) / 1e9
+        rname = 'float' if args.precision == 'fp32' else 'double'
         roof = {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                 "frac": achieved / HBM_PEAK_GBS,
                 "traffic": load_traffic(f'k_discrete_{args.precision}_np9_{n}'),
-                "kernel": "k_discrete<float,9> (fused move+PES+branch+compaction+vref)" if args.precision == 'fp32'
-                else "k_discrete<double,9>",
-                "avg_launch_us": kern_ms * 1e3, "bytes_per_launch": bytes_per_launch,
-                "bytes_per_walker_step": bpw}
-    log(f'[bench] plain: {value:.4g} walker-steps/s, {ms_per_step * 1e3:.2f} us/step, kernel {kern_ms} ms')
+                "kernel": f"k_discrete<{rname},9> (fused move+PES+branch+compaction)",
+                "avg_launch_us": k_ms * 1e3, "bytes_per_launch": bytes_per_launch,
+                "bytes_per_walker_step": bpw,
+                "step_kernels_us": {"k_discrete": kern_ms[0] * 1e3,
+                                    "k_finalize_discrete": kern_ms[1] * 1e3 if len(kern_ms) > 1 else None}}
+    log(f'[bench] plain: {value:.4g} walker-steps/s, {ms_per_step * 1e3:.2f} us/step, kernels {kern_ms} ms')
     del sim
     torch.cuda.empty_cache()
 
@@ -180,10 +188,13 @@ def main():
     if not args.no_variants:
         log('[bench] importance-sampled variant')
         sim = make_sim(n, total, precision=args.precision, imp=True)
-        ws2, wall2, mean2, kern2 = run_timed(sim, W, K, 0)
+        ws2, wall2, mean2, kern2, _ = run_timed(sim, W, K, min(args.event_steps, 50))
         variants['is'] = {"value": ws2 / wall2, "unit": "walker-steps/s", "ms_per_step": wall2 * 1e3 / K,
                           "config": "same PES and walkers, Gaussian-product psi_T (2 stretches + bend), drift and "
                                     "local energy from the trial kernels, Metropolis accept, dt_eff"}
+        if kern2:
+            names = ['k_accept', 'k_acc_finalize', 'k_discrete', 'k_finalize_discrete']
+            variants['is']['step_kernels_us'] = {nm: t * 1e3 for nm, t in zip(names, kern2)}
         log(f'[bench] IS: {ws2 / wall2:.4g} walker-steps/s')
         del sim
         torch.cuda.empty_cache()

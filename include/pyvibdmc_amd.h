@@ -111,6 +111,11 @@ int pvd_sim_start_external(pvd_sim* sim, const double* v_dev, void* stream);
 /* n_steps fused steps with Philox draws: move, PES, weights / branching, vref
  * (pyvibdmc.py:772-848). */
 int pvd_sim_run(pvd_sim* sim, int32_t n_steps, void* stream);
+/* pvd_sim_run with one kernel launch at a time, each dispatch carrying a HIP start / stop
+ * event pair on `stream` (hipExtLaunchKernel); avg_ms[k] (k < *n_kernels <= 8) is the mean
+ * duration of the step's k-th kernel in launch order.  Synchronous.  No counterpart in the
+ * reference (a measurement hook for the per-kernel timings a profiler also reports). */
+int pvd_sim_run_timed(pvd_sim* sim, int32_t n_steps, void* stream, float* avg_ms, int32_t* n_kernels);
 
 /* One step with the caller's draws (device, float64): displacements (n, M, d) as
  * np.random.normal(0, sigma, (n, d, M)).transpose(0, 2, 1) (pyvibdmc.py:544-546),

@@ -65,6 +65,7 @@ _SIGS = {
     'pvd_sim_start': (C.c_int, [_P, _P]),
     'pvd_sim_start_external': (C.c_int, [_P, _P, _P]),
     'pvd_sim_run': (C.c_int, [_P, C.c_int32, _P]),
+    'pvd_sim_run_timed': (C.c_int, [_P, C.c_int32, _P, C.POINTER(C.c_float), C.POINTER(C.c_int32)]),
     'pvd_sim_step_injected': (C.c_int, [_P, _P, _P, _P, _P]),
     'pvd_sim_move': (C.c_int, [_P, _P, _P, _P]),
     'pvd_sim_branch_external': (C.c_int, [_P, _P, _P, _P]),
@@ -84,7 +85,9 @@ _SIGS = {
 
 
 def lib_path():
-    return os.path.join(HERE, LIB_NAME)
+    """The built library: PYVIBDMC_AMD_LIB if set (another build of the same sources),
+    else the in-tree build next to this file."""
+    return os.environ.get('PYVIBDMC_AMD_LIB') or os.path.join(HERE, LIB_NAME)
 
 
 def load():

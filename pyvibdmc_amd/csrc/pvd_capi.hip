@@ -2,12 +2,14 @@
 // auxiliary kernels (layout transposes, first-step vref, descendant weights, stand-alone
 // potential / trial evaluation).  Declarations and conventions: include/pyvibdmc_amd.h.
 #include <hip/hip_runtime.h>
+#include <hip/hip_ext.h>
 
 #include <cmath>
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
 #include <string>
+#include <vector>
 
 #include "../../include/pyvibdmc_amd.h"
 #include "pvd_kernels.hpp"
@@ -241,6 +243,7 @@ unsigned nblk(long long n, int bs = 256) { return (unsigned)((n + bs - 1) / bs);
 struct pvd_sim {
   pvd_config cfg;
   int np, M, d, prec, cont, is;
+  int mspec;  // ModelSpec of the fused step kernels (0 = runtime model parameters)
   long long cap;
   void* coords[2];
   int* who[2];
@@ -321,6 +324,7 @@ StepArgs<R> make_args(pvd_sim* s) {
   for (int p = 0; p < MAXNP; ++p) {
     A.sigma[p] = s->cfg.sigma[p];
     A.inv_m[p] = s->cfg.inv_mass[p];
+    A.h2s[p] = s->cfg.sigma[p] > 0.0 ? (float)(0.5 / (s->cfg.sigma[p] * s->cfg.sigma[p])) : 0.0f;
   }
   A.dt = s->cfg.delta_t;
   A.alpha = s->cfg.alpha;
@@ -339,6 +343,21 @@ StepArgs<R> make_args(pvd_sim* s) {
   return A;
 }
 
+// The compiled model specialisation for a configuration: MS_PES_WATER for the [H, H, O]
+// Morse+bend surface in 3 dimensions, plus MS_TRIAL_WATER when importance sampling uses the
+// matching two-bond / one-angle Gaussian trial.  A water trial on another surface keeps
+// the runtime parameters.
+int model_spec(const pvd_config& c, int np) {
+  if (np != 9 || c.n_atoms != 3 || c.n_dim != 3) return MS_ANY;
+  const pvd_pes_desc& p = c.pes;
+  if (!(p.kind == PES_MORSE_BEND && p.idx[0] == 0 && p.idx[1] == 1 && p.idx[2] == 2)) return MS_ANY;
+  const pvd_trial_desc& t = c.trial;
+  if (t.kind == TRIAL_NONE) return MS_PES_WATER;
+  const bool tw = t.kind == TRIAL_GAUSS && t.nb == 2 && t.na == 1 && t.bi[0] == 0 && t.bj[0] == 2 &&
+                  t.bi[1] == 1 && t.bj[1] == 2 && t.ai[0] == 0 && t.aj[0] == 2 && t.ak[0] == 1;
+  return tw ? (MS_PES_WATER | MS_TRIAL_WATER) : MS_PES_WATER;
+}
+
 // ---- launch helpers, instantiated over the component count ----------------------------
 #define PVD_NP_SWITCH(np, ...)                                  \
   switch (np) {                                                 \
@@ -348,6 +367,24 @@ StepArgs<R> make_args(pvd_sim* s) {
     case 18: { constexpr int NP = 18; __VA_ARGS__; } break;     \
     default: return fail(E_BAD, "fused kernels support 1, 3, 9 or 18 components per walker"); \
   }
+
+// component count and model specialisation (the specialisations exist for 9 components)
+#define PVD_STEP_SWITCH(sim, ...)                                               \
+  PVD_NP_SWITCH((sim)->np, if constexpr (NP == 9) {                             \
+    if ((sim)->mspec == (MS_PES_WATER | MS_TRIAL_WATER)) {                      \
+      constexpr int MS = MS_PES_WATER | MS_TRIAL_WATER;                         \
+      __VA_ARGS__;                                                              \
+    } else if ((sim)->mspec == MS_PES_WATER) {                                  \
+      constexpr int MS = MS_PES_WATER;                                          \
+      __VA_ARGS__;                                                              \
+    } else {                                                                    \
+      constexpr int MS = MS_ANY;                                                \
+      __VA_ARGS__;                                                              \
+    }                                                                           \
+  } else {                                                                      \
+    constexpr int MS = MS_ANY;                                                  \
+    __VA_ARGS__;                                                                \
+  })
 
 // The kernels of one fused step, in launch order, for the current parity.  Used both for
 // direct launches and as the node list of a replayed graph.
@@ -363,26 +400,26 @@ int step_kernels(pvd_sim* s, Launch* L, int& nl) {
   const int pk = s->cfg.pes.kind;
   if (pk == PES_EXTERNAL) return fail(E_BAD, "fused step needs a device potential");
   auto add = [&](const void* fn, unsigned grid, unsigned block) { L[nl++] = Launch{fn, dim3(grid), dim3(block)}; };
-#define PVD_ADD(...) PVD_NP_SWITCH(s->np, add((const void*)(__VA_ARGS__), g, TILE))
+#define PVD_ADD(...) PVD_STEP_SWITCH(s, add((const void*)(__VA_ARGS__), g, TILE))
   if (s->is) {
-    PVD_ADD(k_accept<R, NP, INJ>);
+    PVD_ADD(k_accept<R, NP, INJ, MS>);
     add((const void*)k_acc_finalize<R>, 1, FN_NT);
   }
   if (!s->cont) {
     if (s->is) {
-      PVD_NP_SWITCH(s->np, add((const void*)k_discrete<R, NP, INJ, true, false>,
-                               nblk(s->cap, TILE * disc_wpt<R, NP, false>()), TILE));
+      PVD_STEP_SWITCH(s, add((const void*)k_discrete<R, NP, INJ, true, false, MS>,
+                             nblk(s->cap, TILE * disc_wpt<R, NP, false, true>()), TILE));
       add((const void*)k_finalize_discrete<R, true>, 1, FN_NT);
     } else {
-      PVD_NP_SWITCH(s->np, add((const void*)k_discrete<R, NP, INJ, false, false>,
-                               nblk(s->cap, TILE * disc_wpt<R, NP, false>()), TILE));
+      PVD_STEP_SWITCH(s, add((const void*)k_discrete<R, NP, INJ, false, false, (MS & MS_PES_WATER)>,
+                             nblk(s->cap, TILE * disc_wpt<R, NP, false>()), TILE));
       add((const void*)k_finalize_discrete<R, false>, 1, FN_NT);
     }
   } else {
     if (s->is) {
-      PVD_ADD(k_cont<R, NP, INJ, true, false>);
+      PVD_ADD(k_cont<R, NP, INJ, true, false, MS>);
     } else {
-      PVD_ADD(k_cont<R, NP, INJ, false, false>);
+      PVD_ADD(k_cont<R, NP, INJ, false, false, (MS & MS_PES_WATER)>);
     }
     add((const void*)k_cont_replay<R>, 1, RP_NT);
   }
@@ -485,6 +522,55 @@ int run_steps(pvd_sim* s, int n_steps, const double* disp, const double* um, con
     s->next_step += 1;
     s->epoch_next += 1;
   }
+  return 0;
+}
+
+// Eager steps with an event pair on every kernel dispatch (mean duration per kernel slot).
+template <typename R>
+int run_steps_timed(pvd_sim* s, int n_steps, hipStream_t st, float* avg_ms, int32_t* n_kernels) {
+  Launch L[8];
+  int nl = 0;
+  int rc = step_kernels<R, false>(s, L, nl);
+  if (rc) return rc;
+  std::vector<hipEvent_t> ev((size_t)2 * nl * n_steps, nullptr);
+  auto release = [&]() {
+    for (hipEvent_t e : ev)
+      if (e) (void)hipEventDestroy(e);
+  };
+  for (hipEvent_t& e : ev) {
+    if (hipEventCreate(&e) != hipSuccess) {
+      release();
+      return fail(E_HIP, "hipEventCreate failed");
+    }
+  }
+  for (int k = 0; k < n_steps; ++k) {
+    StepArgs<R> A = make_args<R>(s);
+    void* args[] = {&A};
+    for (int i = 0; i < nl; ++i) {
+      // start / stop events recorded by the dispatch itself (kernel execution only)
+      hipEvent_t a = ev[(size_t)2 * (k * nl + i)], b = ev[(size_t)2 * (k * nl + i) + 1];
+      hipError_t e = hipExtLaunchKernel(L[i].fn, L[i].grid, L[i].block, args, 0, st, a, b, 0);
+      if (e != hipSuccess) {
+        release();
+        return fail(E_HIP, std::string("timed launch: ") + hipGetErrorString(e));
+      }
+    }
+    s->next_step += 1;
+    s->epoch_next += 1;
+  }
+  hipError_t e = hipStreamSynchronize(st);
+  for (int i = 0; i < nl && e == hipSuccess; ++i) {
+    double sum = 0.0;
+    for (int k = 0; k < n_steps && e == hipSuccess; ++k) {
+      float ms = 0.f;
+      e = hipEventElapsedTime(&ms, ev[(size_t)2 * (k * nl + i)], ev[(size_t)2 * (k * nl + i) + 1]);
+      sum += ms;
+    }
+    avg_ms[i] = n_steps > 0 ? (float)(sum / n_steps) : 0.f;
+  }
+  release();
+  if (e != hipSuccess) return fail(E_HIP, std::string("timed launch events: ") + hipGetErrorString(e));
+  *n_kernels = nl;
   return 0;
 }
 
@@ -640,6 +726,7 @@ int pvd_sim_create(const pvd_config* cfg, pvd_sim** out) {
   s->prec = cfg->precision;
   s->cont = cfg->weighting == PVD_CONTINUOUS;
   s->is = cfg->trial.kind != TRIAL_NONE;
+  s->mspec = model_spec(*cfg, np);
   long long cap = cfg->capacity;
   const long long need = (long long)std::floor(1.5 * cfg->num_walkers) + 1;
   if (cap <= 0) cap = need;
@@ -647,6 +734,11 @@ int pvd_sim_create(const pvd_config* cfg, pvd_sim** out) {
   if (cap >= (1ll << 30)) {
     delete s;
     return fail(E_BAD, "capacity must stay below 2^30 walkers");
+  }
+  if (cfg->precision == PVD_FP32 && cap * (long long)np * 4 >= (1ll << 32)) {
+    delete s;
+    return fail(E_BAD, "FP32 walker buffers are addressed with 32-bit offsets: capacity * components * 4 "
+                       "bytes must stay below 4 GiB");
   }
   s->cap = cap;
   s->tmax = cfg->num_timesteps > 0 ? cfg->num_timesteps : 1;
@@ -796,6 +888,13 @@ int pvd_sim_run(pvd_sim* s, int32_t n_steps, void* stream) {
   if (n_steps < 0) return fail(E_BAD, "negative step count");
   return s->prec == PVD_FP64 ? run_steps<double>(s, n_steps, nullptr, nullptr, nullptr, (hipStream_t)stream)
                              : run_steps<float>(s, n_steps, nullptr, nullptr, nullptr, (hipStream_t)stream);
+}
+
+int pvd_sim_run_timed(pvd_sim* s, int32_t n_steps, void* stream, float* avg_ms, int32_t* n_kernels) {
+  if (!s || !avg_ms || !n_kernels) return fail(E_BAD, "null argument");
+  if (n_steps < 0) return fail(E_BAD, "negative step count");
+  return s->prec == PVD_FP64 ? run_steps_timed<double>(s, n_steps, (hipStream_t)stream, avg_ms, n_kernels)
+                             : run_steps_timed<float>(s, n_steps, (hipStream_t)stream, avg_ms, n_kernels);
 }
 
 int pvd_sim_step_injected(pvd_sim* s, const double* disp, const double* um, const double* ub, void* stream) {

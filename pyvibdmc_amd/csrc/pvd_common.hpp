@@ -87,6 +87,17 @@ PVD_DEV void box_muller_f(uint32_t a, uint32_t b, float& z0, float& z1) {
   z0 = r * c;
   z1 = r * s;
 }
+// Box-Muller from a 24-bit radius uniform r24 (u1 = (r24 + 1) 2^-24 in (0, 1]) and a
+// 16-bit angle uniform a16 (u2 = a16 2^-16 in [0, 1)): z0 = rho cos(2 pi u2),
+// z1 = rho sin(2 pi u2), rho = sqrt(-2 ln u1) = sqrt(-2 ln2 log2 u1).  The hardware sine /
+// cosine take the angle in revolutions.
+PVD_DEV void box_muller_24_16(uint32_t r24, uint32_t a16, float& z0, float& z1) {
+  const float u1 = (float)(r24 + 1u) * 5.9604644775390625e-08f;
+  const float u2 = (float)a16 * 1.52587890625e-05f;
+  const float rho = __builtin_amdgcn_sqrtf(-1.3862943611198906f * __builtin_amdgcn_logf(u1));
+  z0 = rho * __builtin_amdgcn_cosf(u2);
+  z1 = rho * __builtin_amdgcn_sinf(u2);
+}
 PVD_DEV void box_muller_d(uint32_t a0, uint32_t a1, uint32_t b0, uint32_t b1, double& z0, double& z1) {
   const double u1 = u01d_open0(a0, a1);
   const double u2 = u01d(b0, b1);

@@ -104,6 +104,7 @@ struct StepArgs {
   TrialParams trial;
   double sigma[MAXNP];
   double inv_m[MAXNP];
+  float h2s[MAXNP];  // 1 / (2 sigma^2), FP32 Metropolis ratio
   double dt, alpha, n0, pop_lo, pop_hi, thresh_lo;
   Rng rng;
   int par;
@@ -113,18 +114,68 @@ struct StepArgs {
 };
 
 // ---------------------------------------------------------------------------- helpers
+// FP32 coordinate buffers are addressed through one buffer resource each: a 32-bit lane
+// offset (4 i) plus a uniform plane offset (4 p cap), so walker accesses carry no 64-bit
+// address arithmetic.  The host keeps np * cap * 4 below 2^32 bytes on this path; the
+// resource's record count bounds every access to the buffer.
+PVD_DEV __amdgpu_buffer_rsrc_t coord_rsrc(const float* base, long long cap, int np) {
+  return __builtin_amdgcn_make_buffer_rsrc(const_cast<float*>(base), 0, (int)(uint32_t)(cap * np * 4),
+                                           0x00020000);
+}
+PVD_DEV float coord_ld(__amdgpu_buffer_rsrc_t r, long long cap, int p, long long i) {
+  return __uint_as_float(__builtin_amdgcn_raw_buffer_load_b32(r, (int)(uint32_t)(i * 4),
+                                                              (int)(uint32_t)(p * cap * 4), 0));
+}
+PVD_DEV void coord_st(__amdgpu_buffer_rsrc_t r, long long cap, int p, long long i, float v) {
+  __builtin_amdgcn_raw_buffer_store_b32(__float_as_uint(v), r, (int)(uint32_t)(i * 4),
+                                        (int)(uint32_t)(p * cap * 4), 0);
+}
+
 template <typename R, int NP>
 PVD_DEV void load_walker(const StepArgs<R>& A, long long i, R* x) {
+  if constexpr (sizeof(R) == 4) {
+    const __amdgpu_buffer_rsrc_t r = coord_rsrc((const float*)A.cin, A.cap, NP);
 #pragma unroll
-  for (int p = 0; p < NP; ++p) x[p] = A.cin[p * A.cap + i];
+    for (int p = 0; p < NP; ++p) x[p] = coord_ld(r, A.cap, p, i);
+  } else {
+#pragma unroll
+    for (int p = 0; p < NP; ++p) x[p] = A.cin[p * A.cap + i];
+  }
+}
+
+// Word layout of one walker's FP32 draws at one step (Philox stream RNG_STEP): NPAIR
+// Box-Muller pairs, each from a 24-bit radius uniform (the high 24 bits of its own word)
+// and a 16-bit angle uniform (16-bit halves of the following words); the 24-bit branching
+// and Metropolis uniforms are assembled from the spare low bytes of the radius words and
+// the unused angle half, with extra words only where those run short.
+template <int NP, bool IS>
+struct DrawLayout {
+  static constexpr int NPAIR = (NP + 1) / 2;
+  static constexpr int NANG = (NPAIR + 1) / 2;              // angle words
+  static constexpr int SPARE = NPAIR + ((NPAIR & 1) ? 2 : 0);  // spare bytes without extras
+  static constexpr int NEED = IS ? 6 : 3;
+  static constexpr int NEXTRA = SPARE >= NEED ? 0 : (NEED - SPARE + 3) / 4;
+  static constexpr int NW = NPAIR + NANG + NEXTRA;
+};
+
+// spare byte b of the layout: radius-word low bytes, then the free angle half, then extras
+template <int NP, bool IS>
+PVD_DEV uint32_t spare_byte(const uint32_t* w, int b) {
+  using L = DrawLayout<NP, IS>;
+  if (b < L::NPAIR) return w[b] & 0xFFu;
+  b -= L::NPAIR;
+  if (L::NPAIR & 1) {
+    if (b < 2) return (w[L::NPAIR + L::NANG - 1] >> (16 + 8 * b)) & 0xFFu;
+    b -= 2;
+  }
+  return (w[L::NPAIR + L::NANG + b / 4] >> (8 * (b % 4))) & 0xFFu;
 }
 
 // All draws of walker slot i at one step: the displacement disp = sigma * z, the
 // Metropolis uniform and the branching uniform.  Injected values are read from the
-// caller's arrays (the reference's draw order).  Philox, FP32: one stream of words per
-// (slot, step) - Box-Muller pairs from consecutive words, then one 24-bit word per
-// uniform (9 normals + 2 uniforms = 3 Philox blocks).  Philox, FP64: 53-bit uniforms from
-// separate blocks per purpose.
+// caller's arrays (the reference's draw order).  Philox, FP32: DrawLayout words from
+// ceil(NW / 4) blocks of one stream per (slot, step) - 2 blocks for 9 components.
+// Philox, FP64: 53-bit uniforms from separate blocks per purpose.
 template <typename R, int NP, bool INJ, bool IS>
 PVD_DEV void draw_step(const StepArgs<R>& A, long long i, unsigned long long step, R* disp, double& um,
                        double& ub, bool want_branch) {
@@ -134,19 +185,26 @@ PVD_DEV void draw_step(const StepArgs<R>& A, long long i, unsigned long long ste
     um = IS ? A.inj_umet[i] : 0.0;
     ub = want_branch ? A.inj_ubr[i] : 0.0;
   } else if constexpr (sizeof(R) == 4) {
-    constexpr int NPAIR = (NP + 1) / 2;
-    constexpr int NW = 2 * NPAIR + (IS ? 1 : 0) + 1;
-    uint32_t w[NW];
-    philox_words<NW>(A.rng, (uint32_t)i, step, w);
+    using L = DrawLayout<NP, IS>;
+    uint32_t w[L::NW];
+    philox_words<L::NW>(A.rng, (uint32_t)i, step, w);
 #pragma unroll
-    for (int k = 0; k < NPAIR; ++k) {
+    for (int k = 0; k < L::NPAIR; ++k) {
+      const uint32_t ang = (w[L::NPAIR + k / 2] >> (16 * (k & 1))) & 0xFFFFu;
       float z0, z1;
-      box_muller_f(w[2 * k], w[2 * k + 1], z0, z1);
+      box_muller_24_16(w[k] >> 8, ang, z0, z1);
       disp[2 * k] = (R)((float)A.sigma[2 * k] * z0);
       if (2 * k + 1 < NP) disp[2 * k + 1] = (R)((float)A.sigma[2 * k + 1] * z1);
     }
-    um = IS ? (double)u01f(w[2 * NPAIR]) : 0.0;
-    ub = (double)u01f(w[NW - 1]);
+    const uint32_t b24 = (spare_byte<NP, IS>(w, 0) << 16) | (spare_byte<NP, IS>(w, 1) << 8) | spare_byte<NP, IS>(w, 2);
+    ub = (double)((float)b24 * 5.9604644775390625e-08f);
+    if constexpr (IS) {
+      const uint32_t m24 =
+          (spare_byte<NP, IS>(w, 3) << 16) | (spare_byte<NP, IS>(w, 4) << 8) | spare_byte<NP, IS>(w, 5);
+      um = (double)((float)m24 * 5.9604644775390625e-08f);
+    } else {
+      um = 0.0;
+    }
   } else {
     R z[NP];
     Normals<R, NP>::draw(A.rng, (uint32_t)i, step, z);
@@ -168,8 +226,18 @@ PVD_DEV double draw_u(const StepArgs<R>& A, long long i, unsigned long long step
 
 // Local kinetic energy -0.5 sum_a sum_c inv_m * d2psi/psi, summed over atoms first then
 // over components (imp_samp.py:49-53).
-template <typename R, int NP>
+template <typename R, int NP, int MS = MS_ANY>
 PVD_DEV R local_kin(const StepArgs<R>& A, const R* sec) {
+  if constexpr (MS != MS_ANY && NP == 9) {
+    // 3 atoms x 3 components, same summation order
+    R tot = R(0);
+#pragma unroll
+    for (int c = 0; c < 3; ++c) {
+      const R s = ((R)A.inv_m[c] * sec[c] + (R)A.inv_m[3 + c] * sec[3 + c]) + (R)A.inv_m[6 + c] * sec[6 + c];
+      tot = (c == 0) ? s : tot + s;
+    }
+    return R(-0.5) * tot;
+  }
   R tot = R(0);
   for (int c = 0; c < A.ds; ++c) {
     R s = R(0);
@@ -189,10 +257,10 @@ PVD_DEV R local_kin(const StepArgs<R>& A, const R* sec) {
 // Importance-sampled move of one walker (pyvibdmc.py:549-612, imp_samp.py:28-47).
 // On return x holds the new position and the function returns the local energy's
 // kinetic part at that position; `acc` tells whether the proposal was accepted.
-template <typename R, int NP, bool INJ>
+template <typename R, int NP, bool INJ, int MS = MS_ANY>
 PVD_DEV R is_move(const StepArgs<R>& A, const R* disp, double u, R* x, bool& acc) {
   TrialOut<R, NP> tx, ty;
-  trial_eval<R, NP>(A.trial, x, tx);
+  trial_eval_s<R, NP, MS>(A.trial, x, tx);
   R y[NP], Dx[NP], Dy[NP];
   const R dt = (R)A.dt;
 #pragma unroll
@@ -200,7 +268,7 @@ PVD_DEV R is_move(const StepArgs<R>& A, const R* disp, double u, R* x, bool& acc
     Dx[p] = (R)A.inv_m[p] * tx.d1[p];
     y[p] = (x[p] + disp[p]) + Dx[p] * dt;
   }
-  trial_eval<R, NP>(A.trial, y, ty);
+  trial_eval_s<R, NP, MS>(A.trial, y, ty);
 #pragma unroll
   for (int p = 0; p < NP; ++p) Dy[p] = (R)A.inv_m[p] * ty.d1[p];
   R met;
@@ -218,7 +286,15 @@ PVD_DEV R is_move(const StepArgs<R>& A, const R* disp, double u, R* x, bool& acc
       const R t2 = pvd_exp<R>((R(-1) * (a2 * a2)) / (R(2) * s2));
       tc[p] = t1 / t2;
     }
-    if (A.ds == 1 && NP == 1) {
+    if constexpr (MS != MS_ANY && NP == 9) {
+      R prod = R(1);
+#pragma unroll
+      for (int c = 0; c < 3; ++c) {
+        const R pc = (tc[c] * tc[3 + c]) * tc[6 + c];
+        prod = (c == 0) ? pc : prod * pc;
+      }
+      met = prod * ratio;
+    } else if (A.ds == 1 && NP == 1) {
       met = tc[0] * ratio;
     } else {
       R prod = R(1);
@@ -242,7 +318,7 @@ PVD_DEV R is_move(const StepArgs<R>& A, const R* disp, double u, R* x, bool& acc
     R la = R(2) * (ty.lnpsi - tx.lnpsi);
 #pragma unroll
     for (int p = 0; p < NP; ++p) {
-      const R inv2s2 = R(0.5) / ((R)A.sigma[p] * (R)A.sigma[p]);
+      const R inv2s2 = (R)A.h2s[p];
       const R a1 = (x[p] - y[p]) - Dy[p] * dt;
       const R a2 = (y[p] - x[p]) - Dx[p] * dt;
       la += (a2 * a2 - a1 * a1) * inv2s2;
@@ -254,7 +330,7 @@ PVD_DEV R is_move(const StepArgs<R>& A, const R* disp, double u, R* x, bool& acc
 #pragma unroll
     for (int p = 0; p < NP; ++p) x[p] = y[p];
   }
-  return local_kin<R, NP>(A, acc ? ty.d2 : tx.d2);
+  return local_kin<R, NP, MS>(A, acc ? ty.d2 : tx.d2);
 }
 
 // pack / unpack a look-back flag: epoch (32) | status (2) | value (30)
@@ -310,23 +386,26 @@ PVD_DEV int look_back(uint64_t* flags, int tile, unsigned epoch, Ctrl* C, bool& 
 // Walkers per thread of the discrete step: a tile is TILE * W walkers, walker
 // (k, thread) of tile b is slot b*TILE*W + k*TILE + thread (coalesced per k).  Larger tiles
 // mean fewer look-back flags and shorter look-back chains.
-template <typename R, int NP, bool EXT>
+template <typename R, int NP, bool EXT, bool IS = false>
 constexpr int disc_wpt() {
-  return EXT ? 4 : ((int)sizeof(R) * NP <= 36 ? 4 : ((int)sizeof(R) * NP <= 72 ? 2 : 1));
+  // importance sampling keeps two trial evaluations per walker in flight: fewer walkers
+  // per thread keep the kernel at more than two waves per SIMD
+  return EXT ? 4 : (IS ? ((int)sizeof(R) * NP <= 36 ? 2 : 1)
+                       : ((int)sizeof(R) * NP <= 36 ? 4 : ((int)sizeof(R) * NP <= 72 ? 2 : 1)));
 }
 
 // MODE: INJ = injected draws; IS = importance sampling (analytic trial); EXT = energies
 // from V_ext (walkers already moved), runtime component count.
 // Tile b is block b.  Workgroups are dispatched in index order within each XCD, so the
 // predecessors a tile waits for are resident or finished; the wait is still bounded.
-template <typename R, int NP, bool INJ, bool IS, bool EXT>
+template <typename R, int NP, bool INJ, bool IS, bool EXT, int MS = MS_ANY>
 __global__ __launch_bounds__(TILE) void k_discrete(StepArgs<R> A) {
-  constexpr int W = disc_wpt<R, NP, EXT>();
+  constexpr int W = disc_wpt<R, NP, EXT, IS>();
   constexpr int TW = TILE * W;
   __shared__ int s_prefix, s_status;
-  __shared__ int s_sc[TILE / PVD_WAVE];
+  __shared__ int s_wt[W][TILE / PVD_WAVE];
   __shared__ double s_d[TILE / PVD_WAVE];
-  __shared__ int s_i[TILE / PVD_WAVE];
+  __shared__ int s_i[TILE / PVD_WAVE], s_i2[TILE / PVD_WAVE], s_i3[TILE / PVD_WAVE];
   const int par = A.par;
   Ctrl* C = A.ctrl;
   const int tile = blockIdx.x;
@@ -360,12 +439,12 @@ __global__ __launch_bounds__(TILE) void k_discrete(StepArgs<R> A) {
         R ke = R(0);
         if constexpr (IS) {
           bool acc;
-          ke = is_move<R, NP, INJ>(A, disp, um, x[k], acc);
+          ke = is_move<R, NP, INJ, MS>(A, disp, um, x[k], acc);
         } else {
 #pragma unroll
           for (int p = 0; p < NP; ++p) x[k][p] = x[k][p] + disp[p];
         }
-        const R v = pes_eval<R, NP>(A.pes, x[k]);
+        const R v = pes_eval_s<R, NP, MS>(A.pes, x[k]);
         E = IS ? (double)(v + ke) : (double)v;
       }
     }
@@ -376,7 +455,7 @@ __global__ __launch_bounds__(TILE) void k_discrete(StepArgs<R> A) {
       if constexpr (sizeof(R) == 4 && !INJ) {
         // FP32 production path: single-precision weight (the FP64 parity path below
         // follows NumPy's double arithmetic)
-        const float wf = expf((float)((-1. * (E - vref)) * dte));
+        const float wf = pvd_exp<float>((float)((-1. * (E - vref)) * dte));
         if (!isfinite(wf) || (double)wf > A.pop_hi + 1.0) {
           bad = 1;
           ck = 0;
@@ -400,19 +479,52 @@ __global__ __launch_bounds__(TILE) void k_discrete(StepArgs<R> A) {
     births += ck > 1 ? ck - 1 : 0;
     deaths += (valid && ck == 0) ? 1 : 0;
   }
-  // exclusive prefix of every walker inside the tile, in slot order
+  // exclusive prefix of every walker inside the tile, in slot order (k-major), and the
+  // tile sums: wave scans and sums, one exchange through LDS, fixed summation order
+  constexpr int NWV = TILE / PVD_WAVE;
+  const int wv = threadIdx.x / PVD_WAVE, ln = lane_id();
+  int incl[W];
+#pragma unroll
+  for (int k = 0; k < W; ++k) incl[k] = wave_incl_scan(c[k]);
+  sce = wave_sum(sce);
+  births = wave_sum(births);
+  deaths = wave_sum(deaths);
+  bad = wave_sum(bad);
+  if (ln == PVD_WAVE - 1) {
+#pragma unroll
+    for (int k = 0; k < W; ++k) s_wt[k][wv] = incl[k];
+  }
+  if (ln == 0) {
+    s_d[wv] = sce;
+    s_i[wv] = births;
+    s_i2[wv] = deaths;
+    s_i3[wv] = bad;
+  }
+  __syncthreads();
   int excl[W];
   int total = 0;
 #pragma unroll
   for (int k = 0; k < W; ++k) {
-    int tk;
-    excl[k] = total + block_excl_scan<TILE>(c[k], s_sc, tk);
+    int before = 0, tk = 0;
+#pragma unroll
+    for (int v = 0; v < NWV; ++v) {
+      before += v < wv ? s_wt[k][v] : 0;
+      tk += s_wt[k][v];
+    }
+    excl[k] = total + before + incl[k] - c[k];
     total += tk;
   }
-  sce = block_sum<TILE>(sce, s_d);
-  births = block_sum<TILE>(births, s_i);
-  deaths = block_sum<TILE>(deaths, s_i);
-  bad = block_sum<TILE>(bad, s_i);
+  sce = s_d[0];
+  births = s_i[0];
+  deaths = s_i2[0];
+  bad = s_i3[0];
+#pragma unroll
+  for (int v = 1; v < NWV; ++v) {
+    sce += s_d[v];
+    births += s_i[v];
+    deaths += s_i2[v];
+    bad += s_i3[v];
+  }
 
   // publish the tile aggregate, then look back for the exclusive prefix
   if (threadIdx.x == 0) {
@@ -455,6 +567,10 @@ __global__ __launch_bounds__(TILE) void k_discrete(StepArgs<R> A) {
         if (dst >= A.cap) break;
         if constexpr (EXT) {
           for (int p = 0; p < A.np; ++p) A.cout[p * A.cap + dst] = A.cin[p * A.cap + i];
+        } else if constexpr (sizeof(R) == 4) {
+          const __amdgpu_buffer_rsrc_t r = coord_rsrc((const float*)A.cout, A.cap, NP);
+#pragma unroll
+          for (int p = 0; p < NP; ++p) coord_st(r, A.cap, p, dst, x[k][p]);
         } else {
 #pragma unroll
           for (int p = 0; p < NP; ++p) A.cout[p * A.cap + dst] = x[k][p];
@@ -544,7 +660,7 @@ __global__ __launch_bounds__(FN_NT) void k_acc_finalize(StepArgs<R> A) {
 // -------------------------------------------------------------- acceptance pre-pass (IS)
 // Counts the accepted proposals of the coming step; the step kernel recomputes the same
 // proposals (same draws, same arithmetic) and uses the count for dt_eff.
-template <typename R, int NP, bool INJ>
+template <typename R, int NP, bool INJ, int MS = MS_ANY>
 __global__ __launch_bounds__(TILE) void k_accept(StepArgs<R> A) {
   __shared__ int s_i[TILE / PVD_WAVE];
   const int par = A.par;
@@ -562,7 +678,7 @@ __global__ __launch_bounds__(TILE) void k_accept(StepArgs<R> A) {
     load_walker<R, NP>(A, i, x);
     draw_step<R, NP, INJ, true>(A, i, step, disp, um, ub, false);
     bool acc;
-    is_move<R, NP, INJ>(A, disp, um, x, acc);
+    is_move<R, NP, INJ, MS>(A, disp, um, x, acc);
     a = acc ? 1 : 0;
   }
   a = block_sum<TILE>(a, s_i);
@@ -636,7 +752,7 @@ __global__ __launch_bounds__(TILE) void k_move(StepArgs<R> A) {
 // ------------------------------------------------------------------------- continuous
 // Weights update in place (pyvibdmc.py:433, 833-837); walkers below the lower threshold
 // are flagged in per-wave masks for the split replay (pyvibdmc.py:436-437, 340-356).
-template <typename R, int NP, bool INJ, bool IS, bool EXT>
+template <typename R, int NP, bool INJ, bool IS, bool EXT, int MS = MS_ANY>
 __global__ __launch_bounds__(TILE) void k_cont(StepArgs<R> A) {
   __shared__ double s_d[TILE / PVD_WAVE];
   __shared__ int s_i[TILE / PVD_WAVE];
@@ -666,14 +782,14 @@ __global__ __launch_bounds__(TILE) void k_cont(StepArgs<R> A) {
       R ke = R(0);
       if constexpr (IS) {
         bool acc;
-        ke = is_move<R, NP, INJ>(A, disp, um, x, acc);
+        ke = is_move<R, NP, INJ, MS>(A, disp, um, x, acc);
       } else {
 #pragma unroll
         for (int p = 0; p < NP; ++p) x[p] = x[p] + disp[p];
       }
 #pragma unroll
       for (int p = 0; p < NP; ++p) A.cout[p * A.cap + i] = x[p];
-      const R v = pes_eval<R, NP>(A.pes, x);
+      const R v = pes_eval_s<R, NP, MS>(A.pes, x);
       E = IS ? (double)(v + ke) : (double)v;
     }
     w = A.w[i] * exp((-1.0 * (E - vref)) * dte);

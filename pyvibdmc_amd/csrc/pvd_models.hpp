@@ -27,8 +27,10 @@ struct PesParams {
 
 template <typename R>
 PVD_DEV R pvd_exp(R x);
+// single precision: the hardware 2^x on x log2(e) (a few ulp; the FP32 surfaces and
+// weights are far less sensitive than that)
 template <>
-PVD_DEV float pvd_exp<float>(float x) { return expf(x); }
+PVD_DEV float pvd_exp<float>(float x) { return __builtin_amdgcn_exp2f(x * 1.4426950408889634f); }
 template <>
 PVD_DEV double pvd_exp<double>(double x) { return exp(x); }
 template <typename R>
@@ -47,6 +49,20 @@ template <>
 PVD_DEV float pvd_log<float>(float x) { return logf(x); }
 template <>
 PVD_DEV double pvd_log<double>(double x) { return log(x); }
+
+// a / b with the reciprocal rb = 1 / b computed once by the caller: single precision
+// multiplies by it; double precision keeps the literal division (the FP64 forms follow
+// the oracle's arithmetic), and the unused reciprocal is dead code there.
+template <typename R>
+PVD_DEV R divr(R a, R b, R rb) {
+  if constexpr (sizeof(R) == 4) {
+    (void)b;
+    return a * rb;
+  } else {
+    (void)rb;
+    return a / b;
+  }
+}
 
 template <typename R>
 PVD_DEV R clamp1(R c) {
@@ -291,12 +307,13 @@ PVD_DEV void bond_term(const R* x, int i, int j, R al, R r0, R* g, R* h, R& q2) 
     u[c] = xi - xj;
   }
   const R r = norm3(u[0], u[1], u[2]);
+  const R rr = R(1) / r;
   const R dq = r - r0;
   q2 += al * (dq * dq);
 #pragma unroll
   for (int c = 0; c < 3; ++c) {
-    const R e = u[c] / r;
-    const R d2 = (R(1) - e * e) / r;
+    const R e = divr(u[c], r, rr);
+    const R d2 = divr(R(1) - e * e, r, rr);
     const R t1 = -al * dq * e;
     const R t2 = -al * (e * e + dq * d2);
 #pragma unroll
@@ -331,30 +348,32 @@ PVD_DEV void angle_term(const R* x, int i, int j, int kk, R al, R th0, R* g, R* 
     v[c] = xk - xj;
   }
   const R ru = norm3(u[0], u[1], u[2]), rv = norm3(v[0], v[1], v[2]);
+  const R rru = R(1) / ru, rrv = R(1) / rv;
   R uh[3], vh[3];
 #pragma unroll
   for (int c = 0; c < 3; ++c) {
-    uh[c] = u[c] / ru;
-    vh[c] = v[c] / rv;
+    uh[c] = divr(u[c], ru, rru);
+    vh[c] = divr(v[c], rv, rrv);
   }
   const R cth = clamp1(uh[0] * vh[0] + uh[1] * vh[1] + uh[2] * vh[2]);
   const R s = pvd_sqrt<R>(R(1) - cth * cth);
+  const R rs = R(1) / s, rs3 = rs * rs * rs;
   const R dq = pvd_acos<R>(cth) - th0;
   q2 += al * (dq * dq);
 #pragma unroll
   for (int c = 0; c < 3; ++c) {
-    const R dcu = (vh[c] - cth * uh[c]) / ru;
-    const R dcv = (uh[c] - cth * vh[c]) / rv;
-    const R duu = (R(-2) * uh[c] * vh[c] - cth + R(3) * cth * uh[c] * uh[c]) / (ru * ru);
-    const R dvv = (R(-2) * uh[c] * vh[c] - cth + R(3) * cth * vh[c] * vh[c]) / (rv * rv);
-    const R duv = (R(1) - uh[c] * uh[c] - vh[c] * vh[c] + cth * uh[c] * vh[c]) / (ru * rv);
+    const R dcu = divr(vh[c] - cth * uh[c], ru, rru);
+    const R dcv = divr(uh[c] - cth * vh[c], rv, rrv);
+    const R duu = divr(R(-2) * uh[c] * vh[c] - cth + R(3) * cth * uh[c] * uh[c], ru * ru, rru * rru);
+    const R dvv = divr(R(-2) * uh[c] * vh[c] - cth + R(3) * cth * vh[c] * vh[c], rv * rv, rrv * rrv);
+    const R duv = divr(R(1) - uh[c] * uh[c] - vh[c] * vh[c] + cth * uh[c] * vh[c], ru * rv, rru * rrv);
     const R dcs[3] = {dcu, dcv, -(dcu + dcv)};
     const R d2cs[3] = {duu, dvv, duu + R(2) * duv + dvv};
     const int at[3] = {i + c, kk + c, j + c};
 #pragma unroll
     for (int z = 0; z < 3; ++z) {
-      const R dth = -dcs[z] / s;
-      const R d2th = -d2cs[z] / s - cth * dcs[z] * dcs[z] / (s * s * s);
+      const R dth = divr(-dcs[z], s, rs);
+      const R d2th = divr(-d2cs[z], s, rs) - divr(cth * dcs[z] * dcs[z], s * s * s, rs3);
       const R t1 = -al * dq * dth;
       const R t2 = -al * (dth * dth + dq * d2th);
 #pragma unroll
@@ -416,6 +435,46 @@ PVD_DEV void trial_eval(const TrialParams& T, const R* x, TrialOut<R, NP>& o) {
   for (int p = 0; p < NP; ++p) {
     o.d1[p] = g[p];
     o.d2[p] = g[p] * g[p] + h[p];
+  }
+}
+
+// Compile-time model specialisations of the step kernels.  The host selects them when the
+// model matches exactly, so the kernels carry no runtime index selects for that layout:
+//   MS_PES_WATER    PES_MORSE_BEND with atoms (h1, h2, o) = (0, 1, 2), 3 atoms x 3 dims
+//   MS_TRIAL_WATER  TRIAL_GAUSS with bonds (0-2), (1-2) and the angle 0-2-1, same layout
+enum ModelSpec : int { MS_ANY = 0, MS_PES_WATER = 1, MS_TRIAL_WATER = 2 };
+
+template <typename R, int NP, int MS>
+PVD_DEV R pes_eval_s(const PesParams& P, const R* x) {
+  if constexpr ((MS & MS_PES_WATER) != 0 && NP == 9) {
+    return morse_bend_v<R>(x, 0, 3, 6, R(P.p[0]), R(P.p[1]), R(P.p[2]), R(0.5 * P.p[3]), R(P.p[4]));
+  } else {
+    return pes_eval<R, NP>(P, x);
+  }
+}
+
+template <typename R, int NP, int MS>
+PVD_DEV void trial_eval_s(const TrialParams& T, const R* x, TrialOut<R, NP>& o) {
+  if constexpr ((MS & MS_TRIAL_WATER) != 0 && NP == 9) {
+    R g[NP], h[NP];
+#pragma unroll
+    for (int p = 0; p < NP; ++p) {
+      g[p] = R(0);
+      h[p] = R(0);
+    }
+    R q2 = R(0);
+    bond_term<R, NP>(x, 0, 6, R(T.balpha[0]), R(T.br0[0]), g, h, q2);
+    bond_term<R, NP>(x, 3, 6, R(T.balpha[1]), R(T.br0[1]), g, h, q2);
+    angle_term<R, NP>(x, 0, 6, 3, R(T.aalpha[0]), R(T.ath0[0]), g, h, q2);
+    o.lnpsi = R(T.c0) - R(0.5) * q2;
+    o.psi = pvd_exp<R>(o.lnpsi);
+#pragma unroll
+    for (int p = 0; p < NP; ++p) {
+      o.d1[p] = g[p];
+      o.d2[p] = g[p] * g[p] + h[p];
+    }
+  } else {
+    trial_eval<R, NP>(T, x, o);
   }
 }
 

@@ -64,3 +64,30 @@ def test_gpu_fp32_tracks_fp64():
     sim.propagate()
     T = 40
     np.testing.assert_allclose(sim._vref_vs_tau[:T], g['vref'][:T], rtol=0, atol=1e-6)
+
+
+def test_large_population_matches_oracle():
+    """1.5M walkers, FP64, the reference's draws injected: 2 930 tiles of 512 walkers, so the
+    compaction's look-back crosses several 512-tile windows.  Two steps match the oracle bit
+    for bit in populations, births, deaths and coordinates (np.repeat order)."""
+    import pyvibdmc_amd as pv
+    from oracle import pes as P
+    from oracle.dmc import LegacyDraws, OracleDMC
+    n = 1_500_000
+    masses = [1837.15, 1837.15, 29156.95]
+    start = P.water_eq()[None] + np.random.RandomState(21).normal(0, 0.05, (n, 3, 3))
+    orc = OracleDMC(potential=P.morse_bend, masses=masses, start_structures=start, num_walkers=n,
+                    num_timesteps=2, delta_t=5, equil_steps=10 ** 9, wfn_every=10 ** 9, draws=LegacyDraws(5))
+    orc.propagate()
+    sim = pv.DMC_Sim(num_walkers=n, num_timesteps=2, atoms=['H', 'H', 'O'], masses=masses, delta_t=5,
+                     potential=pv.MorseBendTriatomic(), start_structures=start, precision='fp64',
+                     rng=LegacyDraws(5), write_files=False, equil_steps=10 ** 9, wfn_every=10 ** 9,
+                     chkpt_every=10 ** 9)
+    sim.propagate()
+    np.testing.assert_array_equal(sim._pop_vs_tau, orc._pop_vs_tau)
+    np.testing.assert_allclose(sim._vref_vs_tau, orc._vref_vs_tau, rtol=0, atol=1e-10)
+    b, d = sim.trajectory_counts(0, 2)
+    np.testing.assert_array_equal(b, orc.rec['births'])
+    np.testing.assert_array_equal(d, orc.rec['deaths'])
+    coords, _ = sim._get_walkers()
+    np.testing.assert_array_equal(coords, orc.coords)
