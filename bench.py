@@ -5,7 +5,7 @@ Workload (the metric's headline case): plain DMC on the triatomic Morse+bend PES
 theta0=104.5 deg), 1M walkers started at the equilibrium geometry + N(0, 0.05 bohr),
 dt = 5 a.u., discrete birth/death, FP32 coordinates and per-walker PES with FP64 weights
 and reductions, Philox draws.  A step is one fused kernel launch (move, PES, weights,
-branching, compaction) and a one-block vref kernel; value = walker-steps/s = sum over the K timed steps of the
+branching, compaction, vref in the last tile); value = walker-steps/s = sum over the K timed steps of the
 population entering each step, divided by the wall time between two device syncs.
 
 Also reported: the same case with importance sampling (Gaussian-product psi_T, drift and
@@ -161,7 +161,7 @@ def main():
     bpw = BYTES_PER_WALKER_STEP[('plain', args.precision, 9)]
     roof = None
     if kern_ms:
-        # plain discrete step = [k_discrete, k_finalize_discrete]; the first dominates
+        # the plain discrete step is one kernel (k_discrete, finalised by its last tile)
         k_ms = kern_ms[0]
         bytes_per_launch = bpw * ev_n
         achieved = bytes_per_launch / (k_ms * 1e-3) / 1e9
@@ -172,8 +172,7 @@ def main():
                 "kernel": f"k_discrete<{rname},9> (fused move+PES+branch+compaction)",
                 "avg_launch_us": k_ms * 1e3, "bytes_per_launch": bytes_per_launch,
                 "bytes_per_walker_step": bpw,
-                "step_kernels_us": {"k_discrete": kern_ms[0] * 1e3,
-                                    "k_finalize_discrete": kern_ms[1] * 1e3 if len(kern_ms) > 1 else None}}
+                "step_kernels_us": {"k_discrete": kern_ms[0] * 1e3}}
     log(f'[bench] plain: {value:.4g} walker-steps/s, {ms_per_step * 1e3:.2f} us/step, kernels {kern_ms} ms')
     del sim
     torch.cuda.empty_cache()
@@ -193,7 +192,7 @@ def main():
                           "config": "same PES and walkers, Gaussian-product psi_T (2 stretches + bend), drift and "
                                     "local energy from the trial kernels, Metropolis accept, dt_eff"}
         if kern2:
-            names = ['k_accept', 'k_acc_finalize', 'k_discrete', 'k_finalize_discrete']
+            names = ['k_is_move', 'k_discrete']
             variants['is']['step_kernels_us'] = {nm: t * 1e3 for nm, t in zip(names, kern2)}
         log(f'[bench] IS: {ws2 / wall2:.4g} walker-steps/s')
         del sim

@@ -255,6 +255,8 @@ struct pvd_sim {
   Ctrl* ctrl;
   uint64_t* flags;
   TileRec* rec;
+  uint64_t* drec;
+  unsigned* accs;
   double* tr_vref;
   double* tr_pop;
   double* tr_dteff;
@@ -308,6 +310,8 @@ StepArgs<R> make_args(pvd_sim* s) {
   A.ctrl = s->ctrl;
   A.flags = s->flags;
   A.rec = s->rec;
+  A.drec = s->drec;
+  A.accs = s->accs;
   A.tr_vref = s->tr_vref;
   A.tr_pop = s->tr_pop;
   A.tr_births = s->tr_births;
@@ -401,19 +405,14 @@ int step_kernels(pvd_sim* s, Launch* L, int& nl) {
   if (pk == PES_EXTERNAL) return fail(E_BAD, "fused step needs a device potential");
   auto add = [&](const void* fn, unsigned grid, unsigned block) { L[nl++] = Launch{fn, dim3(grid), dim3(block)}; };
 #define PVD_ADD(...) PVD_STEP_SWITCH(s, add((const void*)(__VA_ARGS__), g, TILE))
-  if (s->is) {
-    PVD_ADD(k_accept<R, NP, INJ, MS>);
-    add((const void*)k_acc_finalize<R>, 1, FN_NT);
-  }
+  if (s->is) PVD_ADD(k_is_move<R, NP, INJ, MS>);
   if (!s->cont) {
     if (s->is) {
       PVD_STEP_SWITCH(s, add((const void*)k_discrete<R, NP, INJ, true, false, MS>,
                              nblk(s->cap, TILE * disc_wpt<R, NP, false, true>()), TILE));
-      add((const void*)k_finalize_discrete<R, true>, 1, FN_NT);
     } else {
       PVD_STEP_SWITCH(s, add((const void*)k_discrete<R, NP, INJ, false, false, (MS & MS_PES_WATER)>,
                              nblk(s->cap, TILE * disc_wpt<R, NP, false>()), TILE));
-      add((const void*)k_finalize_discrete<R, false>, 1, FN_NT);
     }
   } else {
     if (s->is) {
@@ -588,7 +587,6 @@ int do_move(pvd_sim* s, const double* disp, const double* um, hipStream_t st) {
     } else {
       PVD_NP_SWITCH(s->np, hipLaunchKernelGGL((k_move<R, NP, false, true>), dim3(g), dim3(TILE), 0, st, A));
     }
-    hipLaunchKernelGGL((k_acc_finalize<R>), dim3(1), dim3(FN_NT), 0, st, A);
   } else {
     if (disp) hipLaunchKernelGGL((k_move<R, 1, true, false>), dim3(g), dim3(TILE), 0, st, A);
     else hipLaunchKernelGGL((k_move<R, 1, false, false>), dim3(g), dim3(TILE), 0, st, A);
@@ -611,8 +609,6 @@ int do_branch_ext(pvd_sim* s, const double* v, const double* ub, hipStream_t st)
       if (s->is) hipLaunchKernelGGL((k_discrete<R, 1, false, true, true>), dim3(g), dim3(TILE), 0, st, A);
       else hipLaunchKernelGGL((k_discrete<R, 1, false, false, true>), dim3(g), dim3(TILE), 0, st, A);
     }
-    if (s->is) hipLaunchKernelGGL((k_finalize_discrete<R, true>), dim3(1), dim3(FN_NT), 0, st, A);
-    else hipLaunchKernelGGL((k_finalize_discrete<R, false>), dim3(1), dim3(FN_NT), 0, st, A);
   } else {
     if (s->is) hipLaunchKernelGGL((k_cont<R, 1, false, true, true>), dim3(g), dim3(TILE), 0, st, A);
     else hipLaunchKernelGGL((k_cont<R, 1, false, false, true>), dim3(g), dim3(TILE), 0, st, A);
@@ -689,7 +685,7 @@ void free_sim(pvd_sim* s) {
   for (int i = 0; i < 2; ++i)
     if (s->graph[i]) (void)hipGraphExecDestroy(s->graph[i]);
   void* ptrs[] = {s->coords[0], s->coords[1], s->who[0], s->who[1], s->w, s->E, s->ke, s->kmask,
-                  s->ctrl, s->flags, s->rec, s->tr_vref, s->tr_pop, s->tr_dteff, s->tr_births,
+                  s->ctrl, s->flags, s->rec, s->drec, s->accs, s->tr_vref, s->tr_pop, s->tr_dteff, s->tr_births,
                   s->tr_deaths, s->pend, s->fin, s->klist, s->killtag, s->rstage, s->stage, s->parent, s->parent_w, s->desc,
                   s->cnt, s->dacc};
   for (void* p : ptrs)
@@ -763,6 +759,8 @@ int pvd_sim_create(const pvd_config* cfg, pvd_sim** out) {
   ALLOC(s->ctrl, sizeof(Ctrl));
   ALLOC(s->flags, sizeof(uint64_t) * ntiles);
   ALLOC(s->rec, sizeof(TileRec) * ntiles);
+  ALLOC(s->drec, sizeof(uint64_t) * 3 * ntiles);
+  ALLOC(s->accs, sizeof(unsigned) * 2 * ACC_SLOTS * ACC_STRIDE);
   ALLOC(s->tr_vref, sizeof(double) * s->tmax);
   ALLOC(s->tr_pop, sizeof(double) * s->tmax);
   ALLOC(s->tr_dteff, sizeof(double) * s->tmax);
@@ -778,6 +776,8 @@ int pvd_sim_create(const pvd_config* cfg, pvd_sim** out) {
   ALLOC(s->stage, sizeof(double) * np * cap);
 #undef ALLOC
   hipError_t e = hipMemset(s->flags, 0, sizeof(uint64_t) * ntiles);
+  if (e == hipSuccess) e = hipMemset(s->drec, 0, sizeof(uint64_t) * 3 * ntiles);
+  if (e == hipSuccess) e = hipMemset(s->accs, 0, sizeof(unsigned) * 2 * ACC_SLOTS * ACC_STRIDE);
   if (e == hipSuccess) e = hipMemset(s->ctrl, 0, sizeof(Ctrl));
   if (e == hipSuccess) e = hipMemset(s->tr_vref, 0, sizeof(double) * s->tmax);
   if (e == hipSuccess) e = hipMemset(s->tr_pop, 0, sizeof(double) * s->tmax);
@@ -837,6 +837,7 @@ int pvd_sim_set_walkers(pvd_sim* s, const double* coords, int32_t n, const doubl
   s->epoch_next += 2;
   c.epoch[par] = s->epoch_next;
   HIPCHK(hipMemcpyAsync(s->ctrl, &c, sizeof(c), hipMemcpyHostToDevice, st));
+  HIPCHK(hipMemsetAsync(s->accs, 0, sizeof(unsigned) * 2 * ACC_SLOTS * ACC_STRIDE, st));
   HIPCHK(hipStreamSynchronize(st));
   s->who_on = false;
   return 0;

@@ -271,5 +271,20 @@ PVD_DEV unsigned fetch_add_relaxed(unsigned* p, unsigned v) {
   return __hip_atomic_fetch_add(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
 PVD_DEV void fence_release() { __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent"); }
+PVD_DEV unsigned ld_relaxed_u(const unsigned* p) {
+  return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+PVD_DEV void st_relaxed_u(unsigned* p, unsigned v) {
+  __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+// Wait until this wave's earlier memory operations have completed (relaxed agent-scope
+// stores are then performed at the device's coherence point), with no cache write-back:
+// a later store by this wave cannot be observed before them.  The signal fences keep the
+// compiler from moving memory operations across the wait.
+PVD_DEV void store_complete() {
+  __atomic_signal_fence(__ATOMIC_SEQ_CST);
+  __builtin_amdgcn_s_waitcnt(0);
+  __atomic_signal_fence(__ATOMIC_SEQ_CST);
+}
 
 }  // namespace pvd

@@ -39,7 +39,6 @@ struct Ctrl {
   unsigned epoch[2];
   unsigned ticket[2];
   unsigned done[2];
-  int acc[2];
   int kills[2];
   double sumw[2];
   double sumwe[2];
@@ -49,7 +48,7 @@ struct Ctrl {
   double eff_t;
   unsigned list_n;
   unsigned pad1;
-  int ntd[2];  // tiles of the discrete step with parity p
+  int ntd[2];
 };
 
 struct TileRec {
@@ -86,6 +85,8 @@ struct StepArgs {
   Ctrl* ctrl;
   uint64_t* flags;
   TileRec* rec;
+  uint64_t* drec;      // discrete: 3 coherent words per tile (sum c*E | births,count | epoch,bad,deaths)
+  unsigned* accs;      // importance sampling: accepted-move counters (see acc_slot)
   double* tr_vref;
   double* tr_pop;
   int* tr_births;
@@ -114,6 +115,23 @@ struct StepArgs {
 };
 
 // ---------------------------------------------------------------------------- helpers
+// Accepted-move counters of the importance-sampled step: ACC_SLOTS per parity, one per
+// 128-byte line, so the per-block atomic adds of the acceptance pass spread over many
+// addresses.  Integer sums: the total does not depend on the order of the adds.
+constexpr int ACC_SLOTS = 64;
+constexpr int ACC_STRIDE = 32;  // unsigned words between slots
+
+template <typename R>
+PVD_DEV unsigned* acc_slot(const StepArgs<R>& A, int par, int b) {
+  return A.accs + (size_t)(par * ACC_SLOTS + (b & (ACC_SLOTS - 1))) * ACC_STRIDE;
+}
+
+// Total accepted moves of parity `par`, on every lane of the calling wave.
+template <typename R>
+PVD_DEV unsigned acc_total(const StepArgs<R>& A, int par) {
+  return (unsigned)wave_sum((int)ld_relaxed_u(acc_slot(A, par, lane_id())));
+}
+
 // FP32 coordinate buffers are addressed through one buffer resource each: a 32-bit lane
 // offset (4 i) plus a uniform plane offset (4 p cap), so walker accesses carry no 64-bit
 // address arithmetic.  The host keeps np * cap * 4 below 2^32 bytes on this path; the
@@ -129,6 +147,19 @@ PVD_DEV float coord_ld(__amdgpu_buffer_rsrc_t r, long long cap, int p, long long
 PVD_DEV void coord_st(__amdgpu_buffer_rsrc_t r, long long cap, int p, long long i, float v) {
   __builtin_amdgcn_raw_buffer_store_b32(__float_as_uint(v), r, (int)(uint32_t)(i * 4),
                                         (int)(uint32_t)(p * cap * 4), 0);
+}
+
+// walker i written back into the current (input) buffer
+template <typename R, int NP>
+PVD_DEV void store_walker_in(const StepArgs<R>& A, long long i, const R* x) {
+  if constexpr (sizeof(R) == 4) {
+    const __amdgpu_buffer_rsrc_t r = coord_rsrc((const float*)A.cin, A.cap, NP);
+#pragma unroll
+    for (int p = 0; p < NP; ++p) coord_st(r, A.cap, p, i, x[p]);
+  } else {
+#pragma unroll
+    for (int p = 0; p < NP; ++p) const_cast<R*>(A.cin)[p * A.cap + i] = x[p];
+  }
 }
 
 template <typename R, int NP>
@@ -388,10 +419,104 @@ PVD_DEV int look_back(uint64_t* flags, int tile, unsigned epoch, Ctrl* C, bool& 
 // mean fewer look-back flags and shorter look-back chains.
 template <typename R, int NP, bool EXT, bool IS = false>
 constexpr int disc_wpt() {
-  // importance sampling keeps two trial evaluations per walker in flight: fewer walkers
-  // per thread keep the kernel at more than two waves per SIMD
-  return EXT ? 4 : (IS ? ((int)sizeof(R) * NP <= 36 ? 2 : 1)
-                       : ((int)sizeof(R) * NP <= 36 ? 4 : ((int)sizeof(R) * NP <= 72 ? 2 : 1)));
+  return EXT ? 4 : ((int)sizeof(R) * NP <= 36 ? 4 : ((int)sizeof(R) * NP <= 72 ? 2 : 1));
+}
+
+// Finalise a discrete step inside its last tile: the records of all tiles, each read once
+// its epoch word shows the current step (a tile writes that word after its other words
+// have completed, and before its look-back flag), a fixed-order reduction, the population
+// guards (pyvibdmc.py:397-400, 409-413), calc_vref (pyvibdmc.py:655-661) and the per-step
+// record (pyvibdmc.py:674-682).  `expect` is the population from the look-back, checked
+// against the records.  Called by every thread of the tile.
+template <typename R, bool IS>
+PVD_DEV void finalize_discrete(const StepArgs<R>& A, Ctrl* C, int par, int n, int ntiles, unsigned epoch,
+                               unsigned long long step, double dte, int expect) {
+  __shared__ double f_d[TILE / PVD_WAVE];
+  __shared__ int f_i[TILE / PVD_WAVE];
+  // FQ records per thread and pass, their loads in flight together: one round trip for the
+  // epoch words, one for the sums
+  constexpr int FQ = 8;
+  double s0 = 0.0;
+  int cnt = 0, nb = 0, nd = 0, nbad = 0, late = 0;
+  for (int t0 = 0; t0 < ntiles; t0 += TILE * FQ) {
+    uint64_t w2[FQ];
+#pragma unroll
+    for (int q = 0; q < FQ; ++q) {
+      const int t = t0 + q * TILE + (int)threadIdx.x;
+      w2[q] = t < ntiles ? ld_relaxed(A.drec + 3 * (size_t)t + 2) : ((uint64_t)epoch << 32);
+    }
+#pragma unroll
+    for (int q = 0; q < FQ; ++q) {
+      const int t = t0 + q * TILE + (int)threadIdx.x;
+      long long spins = 0;
+      while ((unsigned)(w2[q] >> 32) != epoch) {
+        if (++spins > (1ll << 26)) {
+          late = 1;
+          break;
+        }
+        __builtin_amdgcn_s_sleep(1);
+        w2[q] = ld_relaxed(A.drec + 3 * (size_t)t + 2);
+      }
+    }
+    __atomic_signal_fence(__ATOMIC_SEQ_CST);  // the sums are read after their epochs matched
+    uint64_t w0[FQ], w1[FQ];
+#pragma unroll
+    for (int q = 0; q < FQ; ++q) {
+      const int t = t0 + q * TILE + (int)threadIdx.x;
+      w0[q] = t < ntiles ? ld_relaxed(A.drec + 3 * (size_t)t) : 0ull;
+      w1[q] = t < ntiles ? ld_relaxed(A.drec + 3 * (size_t)t + 1) : 0ull;
+    }
+#pragma unroll
+    for (int q = 0; q < FQ; ++q) {
+      const int t = t0 + q * TILE + (int)threadIdx.x;
+      if (t < ntiles) {
+        s0 += __longlong_as_double((long long)w0[q]);
+        cnt += (int)(uint32_t)w1[q];
+        nb += (int)(uint32_t)(w1[q] >> 32);
+        nd += (int)(w2[q] & 0xFFFFu);
+        nbad += (int)((w2[q] >> 16) & 0xFFFFu);
+      }
+    }
+  }
+  s0 = block_sum<TILE>(s0, f_d);
+  cnt = block_sum<TILE>(cnt, f_i);
+  nb = block_sum<TILE>(nb, f_i);
+  nd = block_sum<TILE>(nd, f_i);
+  nbad = block_sum<TILE>(nbad, f_i);
+  late = block_sum<TILE>(late, f_i);
+  if (threadIdx.x != 0) return;
+  const int np1 = par ^ 1;
+  int st = ST_OK;
+  if (late || cnt != expect) st = ST_SPIN;
+  else if (nbad) st = ST_WEIGHT;
+  else if ((double)cnt < A.pop_lo || (double)cnt > A.pop_hi) st = ST_POP;
+  else if (cnt > A.cap) st = ST_CAPACITY;
+  if (st != ST_OK) {
+    C->status = st;
+    C->status_step = (long long)step;
+    return;
+  }
+  const double vbar = s0 / (double)cnt;
+  const double corr = (double)(cnt - (int)A.n0) / A.n0;
+  const double vnew = vbar - (A.alpha * corr);
+  C->n[np1] = cnt;
+  C->vref[np1] = vnew;
+  C->step[np1] = step + 1;
+  C->epoch[np1] = epoch + 1;
+  if ((long long)step < A.tmax) {
+    A.tr_vref[step] = vnew;
+    A.tr_pop[step] = (double)cnt;
+    A.tr_births[step] = nb;
+    A.tr_deaths[step] = nd;
+    if (IS) {
+      C->eff_t = (step == 0 ? 0.0 : C->eff_t) + dte;
+      A.tr_dteff[step] = C->eff_t;
+    }
+  }
+  // every tile read the acceptance count before publishing: clear it for step + 2
+  if (IS) {
+    for (int b = 0; b < ACC_SLOTS; ++b) st_relaxed_u(acc_slot(A, par, b), 0u);
+  }
 }
 
 // MODE: INJ = injected draws; IS = importance sampling (analytic trial); EXT = energies
@@ -412,12 +537,11 @@ __global__ __launch_bounds__(TILE) void k_discrete(StepArgs<R> A) {
   const int n = C->n[par];
   const int ntiles = (n + TW - 1) / TW;
   if (C->status != ST_OK || tile >= ntiles) return;
-  if (tile == 0 && threadIdx.x == 0) C->ntd[par] = ntiles;
   const unsigned long long step = C->step[par];
   const unsigned epoch = C->epoch[par];
   const double vref = C->vref[par];
   double dte = A.dt;
-  if (IS) dte = A.dt * ((double)C->acc[par] / (double)n);
+  if (IS) dte = A.dt * ((double)acc_total(A, par) / (double)n);
 
   R x[W][EXT ? 1 : NP];
   int c[W];
@@ -432,26 +556,23 @@ __global__ __launch_bounds__(TILE) void k_discrete(StepArgs<R> A) {
       if constexpr (EXT) {
         E = A.V_ext[i];
         if (IS) E = E + A.ke[i];
+      } else if constexpr (IS) {
+        // moved, and its local energy computed, by k_is_move
+        load_walker<R, NP>(A, i, x[k]);
+        E = A.E[i];
       } else {
         load_walker<R, NP>(A, i, x[k]);
         R disp[NP];
-        draw_step<R, NP, INJ, IS>(A, i, step, disp, um, ub, A.branch != 0);
-        R ke = R(0);
-        if constexpr (IS) {
-          bool acc;
-          ke = is_move<R, NP, INJ, MS>(A, disp, um, x[k], acc);
-        } else {
+        draw_step<R, NP, INJ, false>(A, i, step, disp, um, ub, A.branch != 0);
 #pragma unroll
-          for (int p = 0; p < NP; ++p) x[k][p] = x[k][p] + disp[p];
-        }
-        const R v = pes_eval_s<R, NP, MS>(A.pes, x[k]);
-        E = IS ? (double)(v + ke) : (double)v;
+        for (int p = 0; p < NP; ++p) x[k][p] = x[k][p] + disp[p];
+        E = (double)pes_eval_s<R, NP, MS>(A.pes, x[k]);
       }
     }
     // branching weight and copy count (pyvibdmc.py:392-407)
     int ck = valid ? 1 : 0;
     if (valid && A.branch) {
-      const double u = EXT ? draw_u<R, INJ>(A, i, step, RNG_BRANCH) : ub;
+      const double u = (EXT || IS) ? draw_u<R, INJ>(A, i, step, RNG_BRANCH) : ub;
       if constexpr (sizeof(R) == 4 && !INJ) {
         // FP32 production path: single-precision weight (the FP64 parity path below
         // follows NumPy's double arithmetic)
@@ -526,17 +647,15 @@ __global__ __launch_bounds__(TILE) void k_discrete(StepArgs<R> A) {
     bad += s_i3[v];
   }
 
-  // publish the tile aggregate, then look back for the exclusive prefix
+  // publish the tile record (the epoch word last, after the others have completed), then
+  // the look-back flag, then look back for the exclusive prefix
   if (threadIdx.x == 0) {
-    TileRec r;
-    r.s0 = sce;
-    r.s1 = 0.0;
-    r.mx = 0.0;
-    r.count = total;
-    r.births = births;
-    r.deaths = deaths;
-    r.bad = bad;
-    A.rec[tile] = r;
+    uint64_t* dr = A.drec + 3 * (size_t)tile;
+    st_relaxed(dr + 0, (uint64_t)__double_as_longlong(sce));
+    st_relaxed(dr + 1, (uint64_t)(uint32_t)total | ((uint64_t)(uint32_t)births << 32));
+    store_complete();
+    st_relaxed(dr + 2, ((uint64_t)epoch << 32) | ((uint64_t)(uint32_t)bad << 16) | (uint64_t)(uint32_t)deaths);
+    store_complete();
     st_relaxed(&A.flags[tile], pack_flag(epoch, tile == 0 ? FL_INC : FL_AGG, (unsigned)total));
   }
   if (threadIdx.x < PVD_WAVE) {
@@ -579,89 +698,18 @@ __global__ __launch_bounds__(TILE) void k_discrete(StepArgs<R> A) {
       }
     }
   }
+  // the last tile finishes its look-back after every other tile has published its record
+  if (tile == ntiles - 1) finalize_discrete<R, IS>(A, C, par, n, ntiles, epoch, step, dte, s_prefix + total);
 }
 
-// Finalise a discrete step in one workgroup: fixed-order reduction of the per-tile
-// partials, the population guards (pyvibdmc.py:397-400, 409-413), calc_vref
-// (pyvibdmc.py:655-661) and the per-step record (pyvibdmc.py:674-682).
-constexpr int FN_NT = 1024;
-template <typename R, bool IS>
-__global__ __launch_bounds__(FN_NT) void k_finalize_discrete(StepArgs<R> A) {
-  __shared__ double s_d[FN_NT / PVD_WAVE];
-  __shared__ int s_i[FN_NT / PVD_WAVE];
-  const int par = A.par;
-  Ctrl* C = A.ctrl;
-  if (C->status != ST_OK) return;
-  const int n = C->n[par];
-  const int ntiles = C->ntd[par];
-  const unsigned long long step = C->step[par];
-  double s0 = 0.0;
-  int cnt = 0, nb = 0, nd = 0, nbad = 0;
-  for (int t = threadIdx.x; t < ntiles; t += FN_NT) {
-    const TileRec r = A.rec[t];
-    s0 += r.s0;
-    cnt += r.count;
-    nb += r.births;
-    nd += r.deaths;
-    nbad += r.bad;
-  }
-  s0 = block_sum<FN_NT>(s0, s_d);
-  cnt = block_sum<FN_NT>(cnt, s_i);
-  nb = block_sum<FN_NT>(nb, s_i);
-  nd = block_sum<FN_NT>(nd, s_i);
-  nbad = block_sum<FN_NT>(nbad, s_i);
-  if (threadIdx.x != 0) return;
-  const int np1 = par ^ 1;
-  int st = ST_OK;
-  if (nbad) st = ST_WEIGHT;
-  else if ((double)cnt < A.pop_lo || (double)cnt > A.pop_hi) st = ST_POP;
-  else if (cnt > A.cap) st = ST_CAPACITY;
-  if (st != ST_OK) {
-    C->status = st;
-    C->status_step = (long long)step;
-    return;
-  }
-  const double vbar = s0 / (double)cnt;
-  const double corr = (double)(cnt - (int)A.n0) / A.n0;
-  const double vnew = vbar - (A.alpha * corr);
-  C->n[np1] = cnt;
-  C->vref[np1] = vnew;
-  C->step[np1] = step + 1;
-  C->epoch[np1] = C->epoch[par] + 1;
-  C->ticket[np1] = 0u;
-  if ((long long)step < A.tmax) {
-    A.tr_vref[step] = vnew;
-    A.tr_pop[step] = (double)cnt;
-    A.tr_births[step] = nb;
-    A.tr_deaths[step] = nd;
-    if (IS) {
-      const double dte = A.dt * ((double)C->acc[par] / (double)n);
-      C->eff_t = (step == 0 ? 0.0 : C->eff_t) + dte;
-      A.tr_dteff[step] = C->eff_t;
-    }
-  }
-}
-
-// Total of the per-tile accepted-move counts of the coming step (IS).
-template <typename R>
-__global__ __launch_bounds__(FN_NT) void k_acc_finalize(StepArgs<R> A) {
-  __shared__ int s_i[FN_NT / PVD_WAVE];
-  const int par = A.par;
-  Ctrl* C = A.ctrl;
-  if (C->status != ST_OK) return;
-  const int n = C->n[par];
-  const int ntiles = (n + TILE - 1) / TILE;
-  int tot = 0;
-  for (int t = threadIdx.x; t < ntiles; t += FN_NT) tot += A.rec[t].births;
-  tot = block_sum<FN_NT>(tot, s_i);
-  if (threadIdx.x == 0) C->acc[par] = tot;
-}
-
-// -------------------------------------------------------------- acceptance pre-pass (IS)
-// Counts the accepted proposals of the coming step; the step kernel recomputes the same
-// proposals (same draws, same arithmetic) and uses the count for dt_eff.
+// ------------------------------------------------------------ importance-sampled move
+// The move of an importance-sampled step (pyvibdmc.py:549-612): drift, proposal,
+// Metropolis test, then the potential and local kinetic energy at the final position
+// (pyvibdmc.py:778-786).  The walker is updated in place and its local energy stored; the
+// accepted proposals are counted for dt_eff, which the weights of the same step need
+// (pyvibdmc.py:387-389, 603), so branching runs as the next kernel.
 template <typename R, int NP, bool INJ, int MS = MS_ANY>
-__global__ __launch_bounds__(TILE) void k_accept(StepArgs<R> A) {
+__global__ __launch_bounds__(TILE) void k_is_move(StepArgs<R> A) {
   __shared__ int s_i[TILE / PVD_WAVE];
   const int par = A.par;
   Ctrl* C = A.ctrl;
@@ -678,11 +726,14 @@ __global__ __launch_bounds__(TILE) void k_accept(StepArgs<R> A) {
     load_walker<R, NP>(A, i, x);
     draw_step<R, NP, INJ, true>(A, i, step, disp, um, ub, false);
     bool acc;
-    is_move<R, NP, INJ, MS>(A, disp, um, x, acc);
+    const R ke = is_move<R, NP, INJ, MS>(A, disp, um, x, acc);
     a = acc ? 1 : 0;
+    const R v = pes_eval_s<R, NP, MS>(A.pes, x);
+    A.E[i] = (double)(v + ke);
+    if (acc) store_walker_in<R, NP>(A, i, x);
   }
   a = block_sum<TILE>(a, s_i);
-  if (threadIdx.x == 0) A.rec[tile].births = a;
+  if (threadIdx.x == 0 && a) fetch_add_relaxed(acc_slot(A, par, tile), (unsigned)a);
 }
 
 // ------------------------------------------------------------------ split path: the move
@@ -745,7 +796,7 @@ __global__ __launch_bounds__(TILE) void k_move(StepArgs<R> A) {
   }
   if constexpr (IS) {
     a = block_sum<TILE>(a, s_i);
-    if (threadIdx.x == 0) A.rec[tile].births = a;
+    if (threadIdx.x == 0 && a) fetch_add_relaxed(acc_slot(A, par, tile), (unsigned)a);
   }
 }
 
@@ -765,7 +816,7 @@ __global__ __launch_bounds__(TILE) void k_cont(StepArgs<R> A) {
   const unsigned long long step = C->step[par];
   const double vref = C->vref[par];
   double dte = A.dt;
-  if (IS) dte = A.dt * ((double)C->acc[par] / (double)n);
+  if (IS) dte = A.dt * ((double)acc_total(A, par) / (double)n);
   const long long i = (long long)tile * TILE + threadIdx.x;
   const bool valid = i < n;
   double E = 0.0, w = 0.0;
@@ -774,23 +825,23 @@ __global__ __launch_bounds__(TILE) void k_cont(StepArgs<R> A) {
     if constexpr (EXT) {
       E = A.V_ext[i];
       if (IS) E = E + A.ke[i];
+    } else if constexpr (IS) {
+      // moved, and its local energy computed, by k_is_move
+      R x[NP];
+      load_walker<R, NP>(A, i, x);
+#pragma unroll
+      for (int p = 0; p < NP; ++p) A.cout[p * A.cap + i] = x[p];
+      E = A.E[i];
     } else {
       R x[NP], disp[NP];
       double um, ub;
       load_walker<R, NP>(A, i, x);
-      draw_step<R, NP, INJ, IS>(A, i, step, disp, um, ub, false);
-      R ke = R(0);
-      if constexpr (IS) {
-        bool acc;
-        ke = is_move<R, NP, INJ, MS>(A, disp, um, x, acc);
-      } else {
+      draw_step<R, NP, INJ, false>(A, i, step, disp, um, ub, false);
 #pragma unroll
-        for (int p = 0; p < NP; ++p) x[p] = x[p] + disp[p];
-      }
+      for (int p = 0; p < NP; ++p) x[p] = x[p] + disp[p];
 #pragma unroll
       for (int p = 0; p < NP; ++p) A.cout[p * A.cap + i] = x[p];
-      const R v = pes_eval_s<R, NP, MS>(A.pes, x);
-      E = IS ? (double)(v + ke) : (double)v;
+      E = (double)pes_eval_s<R, NP, MS>(A.pes, x);
     }
     w = A.w[i] * exp((-1.0 * (E - vref)) * dte);
     A.w[i] = w;
@@ -878,6 +929,7 @@ __global__ __launch_bounds__(RP_NT) void k_cont_replay(StepArgs<R> A) {
   // 0. fixed-order reduction of the per-tile sums; without splits this finishes the step
   //    (calc_vref, continuous: pyvibdmc.py:659-661)
   {
+    const unsigned acct = A.is_ ? acc_total(A, par) : 0u;
     double a0 = 0.0, a1 = 0.0;
     int k = 0;
     for (int t = threadIdx.x; t < ntiles; t += RP_NT) {
@@ -897,9 +949,14 @@ __global__ __launch_bounds__(RP_NT) void k_cont_replay(StepArgs<R> A) {
       C->step[np1] = step + 1;
       C->epoch[np1] = C->epoch[par] + 1;
       if (A.is_ && (long long)step < A.tmax) {
-        const double dte = A.dt * ((double)C->acc[par] / (double)n);
+        const double dte = A.dt * ((double)acct / (double)n);
         C->eff_t = (step == 0 ? 0.0 : C->eff_t) + dte;
         A.tr_dteff[step] = C->eff_t;
+      }
+      // the step's weights and every thread here have read the acceptance count (the block
+      // sums above hold barriers): clear it for step + 2
+      if (A.is_) {
+        for (int b = 0; b < ACC_SLOTS; ++b) st_relaxed_u(acc_slot(A, par, b), 0u);
       }
       if (k == 0) {
         const double vnew = a0 / a1 - (A.alpha * ((a1 - A.n0) / A.n0));
