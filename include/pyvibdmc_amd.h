@@ -104,7 +104,8 @@ int pvd_sim_get_walkers(pvd_sim* sim, double* coords, double* weights, int64_t m
                         int32_t* n, void* stream);
 
 /* First-step reference energy: V(x0) [+ local kinetic energy] then calc_vref
- * (pyvibdmc.py:760-769).  The _external form takes V(x0) from device memory. */
+ * (pyvibdmc.py:760-769).  The _external form takes V(x0) from device memory; with a
+ * surrogate set, pvd_sim_start evaluates it. */
 int pvd_sim_start(pvd_sim* sim, void* stream);
 int pvd_sim_start_external(pvd_sim* sim, const double* v_dev, void* stream);
 
@@ -182,6 +183,10 @@ typedef struct pvd_mlp {
   int32_t use_range;       /* 1: walkers outside [lo, hi] use the fallback potential */
   float xmean[32], xinv[32], lo[32], hi[32];
   float ymean, ystd;
+  /* FP32 quadratic baseline in the normalised features x: V = ystd * y + ymean + q(x),
+   * q(x) = quad[0] + sum_i quad[1 + i] x_i + sum_{i <= j} quad[..] x_i x_j (pairs in row
+   * order i = 0..D-1, j = i..D-1); all zero for a plain network. */
+  float quad[136];
   const float* params;     /* device, pvd_mlp_param_count(D, H) floats */
   void* packs;             /* device, pvd_mlp_pack_bytes(H) bytes, written by pvd_mlp_pack */
 } pvd_mlp;
@@ -201,6 +206,13 @@ int pvd_mlp_descriptors(const pvd_mlp* m, int32_t precision, int32_t n_atoms, co
  * else n_max.  `fallback` is the analytic potential used outside the training range. */
 int pvd_mlp_energy(const pvd_mlp* m, int32_t precision, int32_t n_atoms, const void* planes, int64_t stride,
                    const int32_t* n_dev, int64_t n_max, double* v_out, const pvd_pes_desc* fallback, void* stream);
+/* Error judge of a surrogate: the analytic potential on every `every`-th walker of the SoA
+ * planes against v (device, float64).  No counterpart in the reference (its NN potential,
+ * simulation_utilities/potential_manager.py:177-214, is never checked during a run);
+ * record (device, 4 doubles) = {step, sum of squared errors, max |error|, samples}. */
+int pvd_mlp_judge(const pvd_pes_desc* pes, int32_t precision, int32_t n_atoms, const void* planes, int64_t stride,
+                  const int32_t* n_dev, int64_t n_max, const double* v, int32_t every, int64_t step, double* record,
+                  void* stream);
 /* Mean-squared-error gradients (device, flat parameter order) on normalised features
  * x_t (D x n) and targets (n); y_out (n) or NULL; loss_parts (ceil(n/256) doubles). */
 int pvd_mlp_grads(const pvd_mlp* m, const float* x_t, const float* targets, int64_t n, float* grads, float* y_out,
@@ -211,6 +223,17 @@ int pvd_mlp_adam(float* params, float* m1, float* m2, const float* grads, int64_
 /* Device view of the current walkers without a host sync: SoA planes, stride, and a
  * device pointer to the current population. */
 int pvd_sim_device_view(pvd_sim* sim, void** planes, int64_t* stride, int32_t** n_dev);
+/* Use a surrogate network as the potential of a simulation created with an external
+ * potential (the NN_Potential role, simulation_utilities/potential_manager.py:177-214):
+ * pvd_sim_run then moves, evaluates the network on the device (walkers outside the training
+ * range use `fallback`), judges every `judge_every` steps on every `judge_stride`-th walker
+ * (0: no judge) and branches, with no host synchronisation.  The network's device
+ * parameters must stay alive while the simulation uses them; net == NULL removes it. */
+int pvd_sim_set_surrogate(pvd_sim* sim, const pvd_mlp* net, const pvd_pes_desc* fallback, int32_t judge_every,
+                          int32_t judge_stride);
+/* Judge records written so far (each {step, sum sq err, max |err|, samples}); copies up to
+ * max_records into out (host) and sets *n. Synchronous. */
+int pvd_sim_judge_log(pvd_sim* sim, double* out, int32_t max_records, int32_t* n, void* stream);
 
 #ifdef __cplusplus
 }

@@ -13,9 +13,12 @@ descriptor unchanged, the property the reference tests at tests/test_cupy_dist.p
 and by closed-form distances.
 
 Network: x_n = (desc - mean) * inv_std; h0 = silu(W0 x_n + b0); h1 = silu(W1 h0 + b1);
-h2 = silu(W2 h1 + b2); y = w3 . h2 + b3; V = y * y_std + y_mean.  MFMA operands are
+h2 = silu(W2 h1 + b2); y = w3 . h2 + b3; V = y * y_std + y_mean + q(x_n), q a quadratic
+baseline (``quad_terms``; all-zero coefficients for a plain network).  MFMA operands are
 bf16(W_l) and bf16(input of layer l); the output layer runs in FP32 on unrounded h2.
-Pinned by central differences of the loss (``tests/test_oracle_mlp.py``).
+Pinned by central differences of the loss (``tests/test_oracle_mlp.py``); the baseline's
+term order and least-squares fit by an exactly representable quadratic
+(``tests/test_oracle_mlp.py``).
 """
 import numpy as np
 
@@ -66,6 +69,23 @@ def descriptor(cds, pairs, sizes):
         out[:, o:o + s] = -np.sort(-inv[:, o:o + s], axis=1)
         o += s
     return out
+
+
+# ---- quadratic baseline ---------------------------------------------------------------
+def quad_terms(xn):
+    """Terms of the quadratic baseline on normalised features xn (B, D): 1, x_i, then
+    x_i x_j for i <= j in row order (the order of pvd_mlp.quad)."""
+    xn = np.asarray(xn, np.float64)
+    B, D = xn.shape
+    cols = [np.ones(B)] + [xn[:, i] for i in range(D)]
+    cols += [xn[:, i] * xn[:, j] for i in range(D) for j in range(i, D)]
+    return np.stack(cols, axis=1)
+
+
+def fit_quad(xn, v):
+    """Least-squares coefficients of the quadratic baseline (FP64)."""
+    c, *_ = np.linalg.lstsq(quad_terms(xn), np.asarray(v, np.float64), rcond=None)
+    return c
 
 
 # ---- network --------------------------------------------------------------------------

@@ -4,6 +4,7 @@
 #include <hip/hip_runtime.h>
 #include <hip/hip_ext.h>
 
+#include <algorithm>
 #include <cmath>
 #include <cstdio>
 #include <cstdlib>
@@ -280,6 +281,14 @@ struct pvd_sim {
   int status_seen;
   int use_graphs;
   hipGraphExec_t graph[2];  // replayed step graphs, without / with descendant tracking
+  // surrogate potential (external-potential simulations only)
+  int surr_on;
+  pvd_mlp surr;             // network description; its device parameters belong to the caller
+  pvd_pes_desc surr_fb;     // analytic fallback and judge potential
+  int judge_every, judge_stride;
+  double* vsur;             // surrogate V of the current walkers (cap)
+  double* jlog;             // judge records, 4 doubles each
+  int jcap, jn;
 };
 
 namespace {
@@ -482,8 +491,41 @@ int build_graph(pvd_sim* s, hipGraphExec_t* out) {
 }
 
 template <typename R>
+int do_move(pvd_sim* s, const double* disp, const double* um, hipStream_t st);
+template <typename R>
+int do_branch_ext(pvd_sim* s, const double* v, const double* ub, hipStream_t st);
+
+// One step with the surrogate potential: move in place, network energy (+ analytic fallback)
+// on the device planes, judge on judge steps, branch.
+template <typename R>
+int surrogate_step(pvd_sim* s, hipStream_t st) {
+  int rc = do_move<R>(s, nullptr, nullptr, st);
+  if (rc) return rc;
+  const int cb = cur_buf(s);
+  const int32_t* n_dev = &s->ctrl->n[(int)(s->next_step & 1)];
+  const int prec = s->prec == PVD_FP64 ? PVD_FP64 : PVD_FP32;
+  rc = pvd_mlp_energy(&s->surr, prec, s->M, s->coords[cb], s->cap, n_dev, s->cap, s->vsur, &s->surr_fb, st);
+  if (rc) return fail(E_BAD, std::string("surrogate energy: ") + pvd_mlp_last_error());
+  if (s->judge_every > 0 && (s->next_step + 1) % s->judge_every == 0 && s->jn < s->jcap) {
+    rc = pvd_mlp_judge(&s->surr_fb, prec, s->M, s->coords[cb], s->cap, n_dev, s->cap, s->vsur, s->judge_stride,
+                       s->next_step, s->jlog + 4ll * s->jn, st);
+    if (rc) return fail(E_BAD, std::string("surrogate judge: ") + pvd_mlp_last_error());
+    ++s->jn;
+  }
+  return do_branch_ext<R>(s, s->vsur, nullptr, st);
+}
+
+template <typename R>
 int run_steps(pvd_sim* s, int n_steps, const double* disp, const double* um, const double* ub,
               hipStream_t st) {
+  if (s->surr_on) {
+    if (disp) return fail(E_BAD, "injected draws with a surrogate potential: use move / branch_external");
+    for (int k = 0; k < n_steps; ++k) {
+      const int rc = surrogate_step<R>(s, st);
+      if (rc) return rc;
+    }
+    return 0;
+  }
   int k = 0;
   const bool graphs = !disp && s->use_graphs && (s->cfg.branch_every <= 1) && n_steps >= PVD_GRAPH_STEPS + 1;
   if (graphs) {
@@ -687,7 +729,7 @@ void free_sim(pvd_sim* s) {
   void* ptrs[] = {s->coords[0], s->coords[1], s->who[0], s->who[1], s->w, s->E, s->ke, s->kmask,
                   s->ctrl, s->flags, s->rec, s->drec, s->accs, s->tr_vref, s->tr_pop, s->tr_dteff, s->tr_births,
                   s->tr_deaths, s->pend, s->fin, s->klist, s->killtag, s->rstage, s->stage, s->parent, s->parent_w, s->desc,
-                  s->cnt, s->dacc};
+                  s->cnt, s->dacc, s->vsur, s->jlog};
   for (void* p : ptrs)
     if (p) (void)hipFree(p);
   delete s;
@@ -873,6 +915,16 @@ int pvd_sim_get_walkers(pvd_sim* s, double* coords, double* weights, int64_t max
 
 int pvd_sim_start(pvd_sim* s, void* stream) {
   if (!s) return fail(E_BAD, "null argument");
+  if (s->surr_on) {
+    // first-step energies from the surrogate on the device
+    const int cb = cur_buf(s);
+    const int prec = s->prec == PVD_FP64 ? PVD_FP64 : PVD_FP32;
+    const int rc = pvd_mlp_energy(&s->surr, prec, s->M, s->coords[cb], s->cap, &s->ctrl->n[(int)(s->next_step & 1)],
+                                  s->cap, s->vsur, &s->surr_fb, stream);
+    if (rc) return fail(E_BAD, std::string("surrogate energy: ") + pvd_mlp_last_error());
+    return s->prec == PVD_FP64 ? do_start<double>(s, s->vsur, (hipStream_t)stream)
+                               : do_start<float>(s, s->vsur, (hipStream_t)stream);
+  }
   if (s->cfg.pes.kind == PES_EXTERNAL) return fail(E_BAD, "external potential: use pvd_sim_start_external");
   return s->prec == PVD_FP64 ? do_start<double>(s, nullptr, (hipStream_t)stream)
                              : do_start<float>(s, nullptr, (hipStream_t)stream);
@@ -938,6 +990,42 @@ int pvd_sim_device_view(pvd_sim* s, void** planes, int64_t* stride, int32_t** n_
   if (planes) *planes = s->coords[cb];
   if (stride) *stride = s->cap;
   if (n_dev) *n_dev = &s->ctrl->n[(int)(s->next_step & 1)];
+  return 0;
+}
+
+int pvd_sim_set_surrogate(pvd_sim* s, const pvd_mlp* net, const pvd_pes_desc* fallback, int32_t judge_every,
+                          int32_t judge_stride) {
+  if (!s) return fail(E_BAD, "null argument");
+  if (!net) {
+    s->surr_on = 0;
+    return 0;
+  }
+  if (s->cfg.pes.kind != PES_EXTERNAL) return fail(E_BAD, "the simulation has a built-in device potential");
+  if (!fallback) return fail(E_BAD, "the surrogate needs an analytic fallback potential");
+  if (s->d != 3 || (s->M != 3 && s->M != 6)) return fail(E_BAD, "the surrogate supports 3 or 6 atoms in 3-D");
+  if (judge_every < 0 || (judge_every > 0 && judge_stride < 1)) return fail(E_BAD, "bad judge schedule");
+  if (!s->vsur) HIPCHK(hipMalloc((void**)&s->vsur, sizeof(double) * s->cap));
+  if (judge_every > 0 && !s->jlog) {
+    s->jcap = (int)std::min<long long>(1 << 20, s->tmax / judge_every + 2);
+    HIPCHK(hipMalloc((void**)&s->jlog, sizeof(double) * 4 * s->jcap));
+  }
+  s->surr = *net;
+  s->surr_fb = *fallback;
+  s->judge_every = judge_every;
+  s->judge_stride = judge_stride;
+  s->jn = 0;
+  s->surr_on = 1;
+  return 0;
+}
+
+int pvd_sim_judge_log(pvd_sim* s, double* out, int32_t max_records, int32_t* n, void* stream) {
+  if (!s || !n) return fail(E_BAD, "null argument");
+  const int k = std::min(s->jn, std::max(0, (int)max_records));
+  if (k > 0 && out) {
+    HIPCHK(hipMemcpyAsync(out, s->jlog, sizeof(double) * 4 * k, hipMemcpyDeviceToHost, (hipStream_t)stream));
+    HIPCHK(hipStreamSynchronize((hipStream_t)stream));
+  }
+  *n = s->jn;
   return 0;
 }
 

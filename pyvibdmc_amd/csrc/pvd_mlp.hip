@@ -12,6 +12,7 @@
 // registers; the weights are stored pre-permuted to match (pvd_mlp_pack).
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <cstring>
 #include <string>
 
@@ -25,6 +26,7 @@ typedef __attribute__((ext_vector_type(4))) float f32x4;
 
 constexpr int MLP_MAXD = 32;
 constexpr int MLP_MAXP = 15;
+constexpr int MLP_NQ = 1 + MLP_MAXP + MLP_MAXP * (MLP_MAXP + 1) / 2;  // 136
 
 struct MlpNet {
   int D, H;
@@ -35,13 +37,35 @@ struct MlpNet {
   const float* b3;
   int npairs, use_range;
   int pi[MLP_MAXP], pj[MLP_MAXP], grp[MLP_MAXP];
+  int cidx[MLP_MAXP];  // canonical pair index of descriptor slot k
   float xmean[MLP_MAXD], xinv[MLP_MAXD], lo[MLP_MAXD], hi[MLP_MAXD];
   float ymean, ystd;
+  float quad[MLP_NQ];  // quadratic baseline on the normalised features
 };
 
-PVD_DEV float silu_f(float z) { return z / (1.0f + __expf(-z)); }
+// quadratic baseline q(x) of the NPR normalised features (FP32): constant, linear terms,
+// then x_i x_j for i <= j in row order
+template <int NPR>
+PVD_DEV float quad_baseline(const MlpNet& M, const float* xn) {
+  float q = M.quad[0];
+#pragma unroll
+  for (int i = 0; i < NPR; ++i) q += M.quad[1 + i] * xn[i];
+  int o = 1 + NPR;
+#pragma unroll
+  for (int i = 0; i < NPR; ++i)
+#pragma unroll
+    for (int j = i; j < NPR; ++j) q += M.quad[o++] * (xn[i] * xn[j]);
+  return q;
+}
+
+// SiLU and its derivative with the hardware exp2 and reciprocal (about 1 ulp each); the
+// same functions serve training, backward and inference, so the network is consistent
+PVD_DEV float sigmoid_f(float z) {
+  return __builtin_amdgcn_rcpf(1.0f + __builtin_amdgcn_exp2f(-1.4426950408889634f * z));
+}
+PVD_DEV float silu_f(float z) { return z * sigmoid_f(z); }
 PVD_DEV float dsilu_f(float z) {
-  const float s = 1.0f / (1.0f + __expf(-z));
+  const float s = sigmoid_f(z);
   return s * (1.0f + z * (1.0f - s));
 }
 
@@ -49,33 +73,41 @@ PVD_DEV float dsilu_f(float z) {
 PVD_DEV int phys_unit(int c, int g, int j) { return 16 * (2 * c + (j >> 2)) + 4 * g + (j & 3); }
 
 // descriptor of one walker: inverse distances of the pairs, sorted descending inside each
-// group by odd-even transposition (comparisons never cross a group boundary)
+// group by odd-even transposition (comparisons never cross a group boundary).  Distances
+// are computed for all NA(NA-1)/2 pairs in canonical order (0,1), (0,2), ..., (1,2), ...
+// with compile-time coordinate indices; M.cidx maps descriptor slot k to its canonical pair.
 template <typename R, int NP>
 PVD_DEV void mlp_descriptor(const MlpNet& M, const R* x, float* f) {
+  constexpr int NA = NP / 3, NPR = NA * (NA - 1) / 2;
+  static_assert(NPR <= MLP_MAXP, "too many atom pairs");
+  float dc[NPR];
+  {
+    int q = 0;
 #pragma unroll
-  for (int k = 0; k < MLP_MAXP; ++k) {
-    f[k] = 0.0f;
-    if (k < M.npairs) {
-      const int bi = 3 * M.pi[k], bj = 3 * M.pj[k];
-      float d[3];
+    for (int i = 0; i < NA; ++i)
 #pragma unroll
-      for (int c = 0; c < 3; ++c) {
-        R xi = R(0), xj = R(0);
-#pragma unroll
-        for (int p = 0; p < NP; ++p) {
-          if (p == bi + c) xi = x[p];
-          if (p == bj + c) xj = x[p];
-        }
-        d[c] = (float)(xi - xj);
+      for (int j = i + 1; j < NA; ++j) {
+        const float d0 = (float)(x[3 * i] - x[3 * j]);
+        const float d1 = (float)(x[3 * i + 1] - x[3 * j + 1]);
+        const float d2 = (float)(x[3 * i + 2] - x[3 * j + 2]);
+        dc[q++] = 1.0f / __fsqrt_rn(d0 * d0 + d1 * d1 + d2 * d2);
       }
-      f[k] = 1.0f / __fsqrt_rn(d[0] * d[0] + d[1] * d[1] + d[2] * d[2]);
-    }
   }
 #pragma unroll
-  for (int r = 0; r < MLP_MAXP; ++r) {
+  for (int k = 0; k < MLP_MAXP; ++k) {
+    float v = 0.0f;
+    if (k < NPR) {
 #pragma unroll
-    for (int k = (r & 1); k + 1 < MLP_MAXP; k += 2) {
-      if (k + 1 < M.npairs && M.grp[k] == M.grp[k + 1] && f[k] < f[k + 1]) {
+      for (int q = 0; q < NPR; ++q)
+        if (M.cidx[k] == q) v = dc[q];
+    }
+    f[k] = v;
+  }
+#pragma unroll
+  for (int r = 0; r < NPR; ++r) {
+#pragma unroll
+    for (int k = (r & 1); k + 1 < NPR; k += 2) {
+      if (M.grp[k] == M.grp[k + 1] && f[k] < f[k + 1]) {
         const float t = f[k];
         f[k] = f[k + 1];
         f[k + 1] = t;
@@ -84,15 +116,19 @@ PVD_DEV void mlp_descriptor(const MlpNet& M, const R* x, float* f) {
   }
 }
 
-// normalised features 8g..8g+7 of one walker as a B operand chunk
+// normalised features 8g..8g+7 of one walker as a B operand chunk (D = NPR features)
+template <int NPR>
 PVD_DEV bf16x8 input_chunk(const MlpNet& M, const float* f, int g, bool valid) {
+  float fn[NPR];
+#pragma unroll
+  for (int k = 0; k < NPR; ++k) fn[k] = (f[k] - M.xmean[k]) * M.xinv[k];
   bf16x8 v;
 #pragma unroll
   for (int j = 0; j < 8; ++j) {
     float s = 0.0f;
 #pragma unroll
-    for (int k = 0; k < MLP_MAXD; ++k)
-      if (k == 8 * g + j && k < M.D) s = (f[k < MLP_MAXP ? k : 0] - M.xmean[k]) * M.xinv[k];
+    for (int k = j; k < NPR; k += 8)
+      if (k == 8 * g + j) s = fn[k];
     v[j] = (__bf16)(valid ? s : 0.0f);
   }
   return v;
@@ -140,12 +176,14 @@ __global__ __launch_bounds__(256) void k_mlp_forward(MlpNet M, const R* planes, 
   bf16x8 bin[NT][KC];
   bool fbk[NT];
   double vfb[NT];
+  float qb[NT];
 #pragma unroll
   for (int t = 0; t < NT; ++t) {
     const long long w = base + t * 16 + col;
     const bool ok = w < n;
     fbk[t] = false;
     vfb[t] = 0.0;
+    qb[t] = 0.0f;
     if constexpr (MODE == 0) {
       float f[MLP_MAXP];
       R x[NP];
@@ -162,7 +200,14 @@ __global__ __launch_bounds__(256) void k_mlp_forward(MlpNet M, const R* planes, 
           vfb[t] = (double)pes_eval<R, NP>(fb, x);
         }
       }
-      bin0[t][0] = input_chunk(M, f, g, ok);
+      constexpr int NPR = NP / 3 * (NP / 3 - 1) / 2;
+      bin0[t][0] = input_chunk<NPR>(M, f, g, ok);
+      if (g == 0) {
+        float xn[NPR];
+#pragma unroll
+        for (int k = 0; k < NPR; ++k) xn[k] = (f[k] - M.xmean[k]) * M.xinv[k];
+        qb[t] = quad_baseline<NPR>(M, xn);
+      }
     } else {
       bf16x8 v;
 #pragma unroll
@@ -215,7 +260,7 @@ __global__ __launch_bounds__(256) void k_mlp_forward(MlpNet M, const R* planes, 
     if (g == 0 && w < n) {
       const float y = s + M.b3[0];
       if constexpr (MODE == 0) {
-        V[w] = fbk[t] ? vfb[t] : (double)(y * M.ystd + M.ymean);
+        V[w] = fbk[t] ? vfb[t] : (double)(y * M.ystd + M.ymean + qb[t]);
       } else {
         Y[w] = y;
       }
@@ -500,6 +545,16 @@ bool build_net(const pvd_mlp* m, MlpNet& N) {
     N.pi[k] = m->pair_i[k];
     N.pj[k] = m->pair_j[k];
     N.grp[k] = m->pair_group[k];
+    N.cidx[k] = 0;
+  }
+  // the descriptor holds every atom pair once: n_atoms = (1 + sqrt(1 + 8 npairs)) / 2
+  int na = 2;
+  while (na * (na - 1) / 2 < m->npairs) ++na;
+  if (na * (na - 1) / 2 != m->npairs) return false;
+  for (int k = 0; k < m->npairs; ++k) {
+    const int i = std::min(m->pair_i[k], m->pair_j[k]), j = std::max(m->pair_i[k], m->pair_j[k]);
+    if (i < 0 || j >= na || i == j) return false;
+    N.cidx[k] = i * na - i * (i + 1) / 2 + (j - i - 1);
   }
   for (int k = 0; k < 32; ++k) {
     N.xmean[k] = m->xmean[k];
@@ -508,8 +563,207 @@ bool build_net(const pvd_mlp* m, MlpNet& N) {
     N.hi[k] = m->hi[k];
   }
   N.ymean = m->ymean;
+  for (int k = 0; k < MLP_NQ; ++k) N.quad[k] = m->quad[k];
   N.ystd = m->ystd;
   return true;
+}
+
+// ------------------------------------------------------------------------------------
+// Inference on walkers, H = 128: persistent workgroups keep all three weight packs (72 KB),
+// the biases and the output weights in LDS for the whole launch and loop over walker
+// chunks (E_WAVES waves x NT_E tiles of 16 walkers).  Each A fragment read from LDS feeds NT_E MFMAs; the
+// biases enter as the MFMA accumulator input.  Walkers outside the training range get the
+// FALLBACK_MARK bit pattern, which k_mlp_fallback replaces with the analytic potential.
+constexpr int NT_E = 2;
+constexpr int E_WAVES = 8;  // waves per workgroup sharing one LDS copy of the weights
+constexpr unsigned long long FALLBACK_MARK = 0x7FF4D1CEFA11BAC4ull;  // a signalling-NaN payload
+
+template <int H>
+constexpr size_t energy_lds_bytes() {
+  return (size_t)((H / 16) * 64 + 2 * (H / 16) * (H / 32) * 64) * 16 + (size_t)(4 * H + 4) * 4;
+}
+
+template <typename R, int NP, int H>
+__global__ __launch_bounds__(64 * E_WAVES, 2) void k_mlp_energy(MlpNet M, const R* planes, long long stride, const int* n_dev,
+                                                       long long n_fix, double* V) {
+  extern __shared__ bf16x8 lds[];
+  constexpr int MT = H / 16, KC = H / 32;
+  bf16x8* L0 = lds;
+  bf16x8* L1 = L0 + MT * 64;
+  bf16x8* L2 = L1 + MT * KC * 64;
+  float* lb = (float*)(L2 + MT * KC * 64);  // b0 | b1 | b2 | w3 | b3
+  for (int q = threadIdx.x; q < MT * 64; q += 64 * E_WAVES) L0[q] = M.Wf[0][q];
+  for (int q = threadIdx.x; q < MT * KC * 64; q += 64 * E_WAVES) {
+    L1[q] = M.Wf[1][q];
+    L2[q] = M.Wf[2][q];
+  }
+  for (int q = threadIdx.x; q < H; q += 64 * E_WAVES) {
+    lb[q] = M.b[0][q];
+    lb[H + q] = M.b[1][q];
+    lb[2 * H + q] = M.b[2][q];
+    lb[3 * H + q] = M.w3[q];
+  }
+  if (threadIdx.x == 0) lb[4 * H] = M.b3[0];
+  __syncthreads();
+  const int n = n_dev ? *n_dev : (int)n_fix;
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const int g = lane >> 4, col = lane & 15;
+  constexpr long long CHUNK = (long long)E_WAVES * 16 * NT_E;
+  for (long long c0 = (long long)blockIdx.x * CHUNK; c0 < n; c0 += (long long)gridDim.x * CHUNK) {
+    // compiler-only barrier: the weight fragments are re-read from LDS each chunk instead of
+    // being hoisted out of the loop into (spilled) registers
+    asm volatile("" ::: "memory");
+    const long long base = c0 + (long long)wave * 16 * NT_E;
+    constexpr int NPR = NP / 3 * (NP / 3 - 1) / 2;
+    bf16x8 bin0[NT_E][1];
+    bf16x8 bin[NT_E][KC];
+    bool fbk[NT_E];
+    float qb[NT_E];
+#pragma unroll
+    for (int t = 0; t < NT_E; ++t) {
+      const long long w = base + t * 16 + col;
+      const bool ok = w < n;
+      float f[MLP_MAXP];
+      R x[NP];
+#pragma unroll
+      for (int p = 0; p < NP; ++p) x[p] = ok ? planes[p * stride + w] : R(0);
+      mlp_descriptor<R, NP>(M, x, f);
+      bool out = false;
+      if (M.use_range) {
+#pragma unroll
+        for (int k = 0; k < NPR; ++k) out |= !(f[k] >= M.lo[k] && f[k] <= M.hi[k]);
+      }
+      fbk[t] = out;
+      bin0[t][0] = input_chunk<NPR>(M, f, g, ok);
+      qb[t] = 0.0f;
+      if (g == 0) {
+        float xn[NPR];
+#pragma unroll
+        for (int k = 0; k < NPR; ++k) xn[k] = (f[k] - M.xmean[k]) * M.xinv[k];
+        qb[t] = quad_baseline<NPR>(M, xn);
+      }
+    }
+    f32x4 acc[NT_E][MT];
+    float part[NT_E];
+#pragma unroll
+    for (int layer = 0; layer < 3; ++layer) {
+      asm volatile("" ::: "memory");
+      const float* bl = lb + layer * H;
+#pragma unroll
+      for (int m = 0; m < MT; ++m) {
+        const f32x4 bias = *(const f32x4*)(bl + 16 * m + 4 * g);
+        if (layer == 0) {
+          const bf16x8 a = L0[m * 64 + lane];
+#pragma unroll
+          for (int t = 0; t < NT_E; ++t) acc[t][m] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, bin0[t][0], bias, 0, 0, 0);
+        } else {
+          const bf16x8* Wl = layer == 1 ? L1 : L2;
+#pragma unroll
+          for (int c = 0; c < KC; ++c) {
+            const bf16x8 a = Wl[(m * KC + c) * 64 + lane];
+#pragma unroll
+            for (int t = 0; t < NT_E; ++t)
+              acc[t][m] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, bin[t][c], c == 0 ? bias : acc[t][m], 0, 0, 0);
+          }
+        }
+      }
+      if (layer < 2) {
+#pragma unroll
+        for (int t = 0; t < NT_E; ++t)
+#pragma unroll
+          for (int m = 0; m < MT; ++m)
+#pragma unroll
+            for (int i = 0; i < 4; ++i) bin[t][m >> 1][(m & 1) * 4 + i] = (__bf16)silu_f(acc[t][m][i]);
+      } else {
+#pragma unroll
+        for (int t = 0; t < NT_E; ++t) {
+          float s = 0.0f;
+#pragma unroll
+          for (int m = 0; m < MT; ++m) {
+            const f32x4 w3 = *(const f32x4*)(lb + 3 * H + 16 * m + 4 * g);
+#pragma unroll
+            for (int i = 0; i < 4; ++i) s += w3[i] * silu_f(acc[t][m][i]);
+          }
+          part[t] = s;
+        }
+      }
+    }
+    const float b3 = lb[4 * H];
+#pragma unroll
+    for (int t = 0; t < NT_E; ++t) {
+      float s = part[t];
+      s += __shfl_xor(s, 16, 64);
+      s += __shfl_xor(s, 32, 64);
+      const long long w = base + t * 16 + col;
+      if (g == 0 && w < n) {
+        const float y = s + b3;
+        V[w] = fbk[t] ? __longlong_as_double((long long)FALLBACK_MARK) : (double)(y * M.ystd + M.ymean + qb[t]);
+      }
+    }
+  }
+}
+
+// analytic potential for the walkers the surrogate marked as outside its training range
+template <typename R, int NP>
+__global__ __launch_bounds__(256) void k_mlp_fallback(const R* planes, long long stride, const int* n_dev,
+                                                      long long n_fix, double* V, PesParams fb) {
+  const int n = n_dev ? *n_dev : (int)n_fix;
+  for (long long w = (long long)blockIdx.x * 256 + threadIdx.x; w < n; w += (long long)gridDim.x * 256) {
+    if ((unsigned long long)__double_as_longlong(V[w]) != FALLBACK_MARK) continue;
+    R x[NP];
+#pragma unroll
+    for (int p = 0; p < NP; ++p) x[p] = planes[p * stride + w];
+    V[w] = (double)pes_eval<R, NP>(fb, x);
+  }
+}
+
+// Error judge: analytic potential on every `every`-th walker against the surrogate's V
+// (one workgroup, fixed-order reduction); record = {step, sum of squared errors, max |error|,
+// samples} in hartree.
+template <typename R, int NP>
+__global__ __launch_bounds__(1024) void k_mlp_judge(const R* planes, long long stride, const int* n_dev,
+                                                    long long n_fix, const double* V, int every, PesParams pes,
+                                                    long long step, double* rec) {
+  __shared__ double s_a[16], s_b[16];
+  __shared__ int s_c[16];
+  const int n = n_dev ? *n_dev : (int)n_fix;
+  double sse = 0.0, mx = 0.0;
+  int cnt = 0;
+  for (long long w = (long long)threadIdx.x * every; w < n; w += 1024ll * every) {
+    R x[NP];
+#pragma unroll
+    for (int p = 0; p < NP; ++p) x[p] = planes[p * stride + w];
+    const double d = V[w] - (double)pes_eval<R, NP>(pes, x);
+    sse += d * d;
+    mx = fmax(mx, fabs(d));
+    ++cnt;
+  }
+  const int wv = threadIdx.x >> 6;
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) {
+    sse += __shfl_xor(sse, o, 64);
+    mx = fmax(mx, __shfl_xor(mx, o, 64));
+    cnt += __shfl_xor(cnt, o, 64);
+  }
+  if ((threadIdx.x & 63) == 0) {
+    s_a[wv] = sse;
+    s_b[wv] = mx;
+    s_c[wv] = cnt;
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    double a = 0.0, b = 0.0;
+    int c = 0;
+    for (int k = 0; k < 16; ++k) {
+      a += s_a[k];
+      b = fmax(b, s_b[k]);
+      c += s_c[k];
+    }
+    rec[0] = (double)step;
+    rec[1] = a;
+    rec[2] = b;
+    rec[3] = (double)c;
+  }
 }
 
 constexpr int NT_FWD = 2;
@@ -517,6 +771,26 @@ constexpr int NT_FWD = 2;
 template <typename R, int NP, int H>
 int launch_energy(const MlpNet& N, const void* planes, long long stride, const int* n_dev, long long n_max,
                   double* V, const PesParams& fb, hipStream_t st) {
+  if constexpr (H == 128) {
+    // persistent: two workgroups per CU (LDS-bound), each looping over walker chunks
+    int dev = 0, cus = 256;
+    MHIP(hipGetDevice(&dev));
+    MHIP(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev));
+    const long long chunks = (n_max + (long long)E_WAVES * 16 * NT_E - 1) / ((long long)E_WAVES * 16 * NT_E);
+    const unsigned grid = (unsigned)std::max(1ll, std::min(chunks, 2ll * cus));
+    const size_t lds = energy_lds_bytes<H>();
+    auto k = k_mlp_energy<R, NP, H>;
+    MHIP(hipFuncSetAttribute((const void*)k, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
+    hipLaunchKernelGGL(k, dim3(grid), dim3(64 * E_WAVES), lds, st, N, (const R*)planes, stride, n_dev, n_max, V);
+    MHIP(hipGetLastError());
+    if (N.use_range) {
+      const unsigned gf = (unsigned)std::max(1ll, std::min((n_max + 255) / 256, 4ll * cus));
+      hipLaunchKernelGGL((k_mlp_fallback<R, NP>), dim3(gf), dim3(256), 0, st, (const R*)planes, stride, n_dev, n_max,
+                         V, fb);
+      MHIP(hipGetLastError());
+    }
+    return 0;
+  }
   const long long per_block = 4ll * 16 * NT_FWD;
   const unsigned grid = (unsigned)((n_max + per_block - 1) / per_block);
   const size_t lds = (size_t)(H / 16) * (H / 32) * 64 * 16;
@@ -567,6 +841,8 @@ int pvd_mlp_descriptors(const pvd_mlp* m, int32_t precision, int32_t n_atoms, co
                         int32_t normalise, float* features_t, void* stream) {
   MlpNet N;
   if (!m || !coords || !features_t || !build_net(m, N)) return mfail(7, "bad network description");
+  if (n_atoms * (n_atoms - 1) / 2 != m->npairs || m->D != m->npairs)
+    return mfail(7, "the descriptor holds every atom pair of the walker once");
   if (n <= 0) return 0;
   hipStream_t st = (hipStream_t)stream;
   const unsigned grid = (unsigned)((n + 255) / 256);
@@ -591,6 +867,8 @@ int pvd_mlp_energy(const pvd_mlp* m, int32_t precision, int32_t n_atoms, const v
                    const int32_t* n_dev, int64_t n_max, double* v_out, const pvd_pes_desc* fallback, void* stream) {
   MlpNet N;
   if (!m || !planes || !v_out || !build_net(m, N)) return mfail(7, "bad network description");
+  if (n_atoms * (n_atoms - 1) / 2 != m->npairs || m->D != m->npairs)
+    return mfail(7, "the descriptor holds every atom pair of the walker once");
   if (n_max <= 0) return 0;
   PesParams fb;
   std::memset(&fb, 0, sizeof(fb));
@@ -612,6 +890,31 @@ int pvd_mlp_energy(const pvd_mlp* m, int32_t precision, int32_t n_atoms, const v
   }
 #undef PVD_E
   return mfail(7, "surrogate supports 3 or 6 atoms and hidden width 128 or 256");
+}
+
+int pvd_mlp_judge(const pvd_pes_desc* pes, int32_t precision, int32_t n_atoms, const void* planes, int64_t stride,
+                  const int32_t* n_dev, int64_t n_max, const double* v, int32_t every, int64_t step, double* record,
+                  void* stream) {
+  if (!pes || !planes || !v || !record || every < 1) return mfail(7, "bad arguments");
+  PesParams P;
+  std::memcpy(&P, pes, sizeof(P));
+  hipStream_t st = (hipStream_t)stream;
+  const int np = n_atoms * 3;
+#define PVD_J(R, NP)                                                                                 \
+  hipLaunchKernelGGL((k_mlp_judge<R, NP>), dim3(1), dim3(1024), 0, st, (const R*)planes, stride, n_dev, n_max, v, \
+                     every, P, step, record)
+  if (precision == PVD_FP64) {
+    if (np == 9) PVD_J(double, 9);
+    else if (np == 18) PVD_J(double, 18);
+    else return mfail(7, "judge supports 3 or 6 atoms");
+  } else {
+    if (np == 9) PVD_J(float, 9);
+    else if (np == 18) PVD_J(float, 18);
+    else return mfail(7, "judge supports 3 or 6 atoms");
+  }
+#undef PVD_J
+  MHIP(hipGetLastError());
+  return 0;
 }
 
 // Gradients of the mean squared error of the network on (X, T) into grads (flat, the

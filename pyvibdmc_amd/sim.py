@@ -242,6 +242,10 @@ class DMC_Sim:
         self._h = h
         self._np = np_
         self._shape = (m, d)
+        # a surrogate network runs inside the library's step loop (no per-step host work)
+        self._surrogate = hasattr(self.potential_obj, 'attach') and pes is not None and pes.kind == N.PES_EXTERNAL
+        if self._surrogate:
+            self.potential_obj.attach(self)
         coords = np.ascontiguousarray(coords, dtype=np.float64)
         w = None if cont_wts is None else np.ascontiguousarray(cont_wts, dtype=np.float64)
         N.check(lib.pvd_sim_set_walkers(self._h, N.ptr(coords), len(coords), N.ptr(w), int(step0),
@@ -350,7 +354,7 @@ class DMC_Sim:
     def _start(self):
         lib = N.load()
         dev_e = getattr(self.potential_obj, 'device_energy', None)
-        if self._fused:
+        if self._fused or self._surrogate:
             N.check(lib.pvd_sim_start(self._h, N.stream_handle()), 'pvd_sim_start')
         elif dev_e is not None:
             v = dev_e(self)
@@ -400,21 +404,17 @@ class DMC_Sim:
         draws = self._draws()
         dev_e = getattr(self.potential_obj, 'device_energy', None)
         lib = N.load()
-        if draws is None and self._fused:
+        if draws is None and (self._fused or self._surrogate):
             N.check(lib.pvd_sim_run(self._h, t1 - t0, N.stream_handle()), 'pvd_sim_run')
             self._raise_if_failed()
         elif draws is None and dev_e is not None:
-            # device potential outside the fused kernel (surrogate): move, V on the device,
+            # another device potential outside the fused kernel: move, V on the device,
             # branch - no host synchronisation inside the chunk
-            every = getattr(self.potential_obj, 'judge_every', 0) or 0
             st = N.stream_handle()
             for t in range(t0, t1):
                 N.check(lib.pvd_sim_move(self._h, None, None, st), 'pvd_sim_move')
                 v = dev_e(self)
                 N.check(lib.pvd_sim_branch_external(self._h, N.ptr(v), None, st), 'pvd_sim_branch_external')
-                if every and (t + 1) % every == 0:
-                    self._raise_if_failed()
-                    self.potential_obj.judge(self, t)
             self._raise_if_failed()
         else:
             for t in range(t0, t1):

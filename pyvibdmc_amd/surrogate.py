@@ -27,7 +27,7 @@ class MLPDesc(ctypes.Structure):
                 ('use_range', ctypes.c_int32),
                 ('xmean', ctypes.c_float * 32), ('xinv', ctypes.c_float * 32), ('lo', ctypes.c_float * 32),
                 ('hi', ctypes.c_float * 32), ('ymean', ctypes.c_float), ('ystd', ctypes.c_float),
-                ('params', ctypes.c_void_p), ('packs', ctypes.c_void_p)]
+                ('quad', ctypes.c_float * 136), ('params', ctypes.c_void_p), ('packs', ctypes.c_void_p)]
 
 
 _P = ctypes.c_void_p
@@ -46,6 +46,11 @@ _MSIGS = {
                                     ctypes.c_float, ctypes.c_float, ctypes.c_int64, _P]),
     'pvd_sim_device_view': (ctypes.c_int, [_P, ctypes.POINTER(_P), ctypes.POINTER(ctypes.c_int64),
                                            ctypes.POINTER(_P)]),
+    'pvd_mlp_judge': (ctypes.c_int, [ctypes.POINTER(N.PesDesc), ctypes.c_int32, ctypes.c_int32, _P, ctypes.c_int64,
+                                     _P, ctypes.c_int64, _P, ctypes.c_int32, ctypes.c_int64, _P, _P]),
+    'pvd_sim_set_surrogate': (ctypes.c_int, [_P, ctypes.POINTER(MLPDesc), ctypes.POINTER(N.PesDesc), ctypes.c_int32,
+                                             ctypes.c_int32]),
+    'pvd_sim_judge_log': (ctypes.c_int, [_P, _P, ctypes.c_int32, ctypes.POINTER(ctypes.c_int32), _P]),
 }
 
 
@@ -212,13 +217,49 @@ class SurrogateModel:
                                    N.stream_handle()), 'pvd_mlp_adam')
         self.repack()
 
-    def fit(self, coords, energies, epochs=200, batch=8192, lr=1e-3, seed=0, verbose=False, max_seconds=None):
+    def quad_design(self, xn):
+        """Design matrix (n, 1 + D + D(D+1)/2) of the quadratic baseline on normalised
+        features xn (D, n): constant, x_i, then x_i x_j for i <= j in row order."""
+        import torch
+        x = xn.double()
+        cols = [torch.ones(x.shape[1], dtype=torch.float64, device=x.device)]
+        cols += [x[i] for i in range(self.D)]
+        cols += [x[i] * x[j] for i in range(self.D) for j in range(i, self.D)]
+        return torch.stack(cols, dim=1)
+
+    def fit_baseline(self, xn, energies):
+        """Least-squares quadratic baseline (FP64 normal equations on the device); returns
+        the residual energies the network is trained on."""
+        import torch
+        e = energies.double()
+        A = self.quad_design(xn)
+        G = A.t() @ A
+        G += torch.eye(G.shape[0], dtype=torch.float64, device=G.device) * (1e-12 * float(G.diagonal().mean()))
+        c = torch.linalg.solve(G, A.t() @ e)
+        cq = c.cpu().numpy()
+        for k in range(136):
+            self.desc.quad[k] = float(cq[k]) if k < len(cq) else 0.0
+        # residual against the baseline as the kernels evaluate it (FP32 coefficients)
+        cf = torch.from_numpy(cq.astype(np.float32).astype(np.float64)).to(A.device)
+        return e - A @ cf
+
+    def fit(self, coords, energies, epochs=200, batch=8192, lr=1e-3, seed=0, verbose=False, max_seconds=None,
+            baseline=True):
         """Adam on the mean squared error of the normalised energies.  Data: (N, M, 3) walkers
-        and (N,) energies in hartree.  Returns the per-epoch training loss (normalised units)."""
+        and (N,) energies in hartree.  With ``baseline`` the network learns the residual of a
+        least-squares quadratic in the normalised descriptor (added back in FP32 at inference),
+        which shrinks the target scale and with it the BF16 rounding error of the output.
+        Returns the per-epoch training loss (normalised units)."""
         import torch
         self.set_normalisation(coords, energies)
         x = self.features(coords, normalise=True)
         e = torch.from_numpy(np.asarray(energies, np.float64)).cuda()
+        for k in range(136):
+            self.desc.quad[k] = 0.0
+        if baseline:
+            e = self.fit_baseline(x, e)
+            self.desc.ymean = float(e.mean())
+            self.desc.ystd = float(e.std())
         t = ((e - self.desc.ymean) / self.desc.ystd).float()
         n = t.shape[0]
         g = torch.Generator(device='cuda')
@@ -268,7 +309,10 @@ class SurrogatePotential(DevicePotential):
     """A trained ``SurrogateModel`` as a DMC potential.  Walkers whose descriptor leaves the
     training range use ``fallback`` (an analytic device potential); every ``judge_every``
     steps the analytic potential is evaluated on every ``judge_stride``-th walker and the
-    surrogate's RMS and maximum error there are recorded in ``judge_log``."""
+    surrogate's error there is recorded (``judge_log``: (step, rms, max |error|, samples),
+    hartree).  Inside ``DMC_Sim`` the library runs move, network, judge and branching on the
+    device with no host synchronisation; ``attach`` copies the network description into
+    the simulation, so a model trained further must be attached again."""
 
     def __init__(self, model, fallback, use_range=True, judge_every=100, judge_stride=100):
         super().__init__()
@@ -280,8 +324,29 @@ class SurrogatePotential(DevicePotential):
         self.model.desc.use_range = 1 if use_range else 0
         self.judge_every = judge_every
         self.judge_stride = judge_stride
-        self.judge_log = []
         self._V = None
+        self._sim = None
+
+    def attach(self, sim):
+        """Register the network with the simulation's library handle."""
+        _check(_lib().pvd_sim_set_surrogate(sim._h, ctypes.byref(self.model.desc), ctypes.byref(self.fallback.desc),
+                                            int(self.judge_every or 0), int(self.judge_stride or 1)),
+               'pvd_sim_set_surrogate')
+        self._sim = sim
+
+    @property
+    def judge_log(self):
+        if self._sim is None or getattr(self._sim, '_h', None) is None:
+            return []
+        lib = _lib()
+        n = ctypes.c_int32()
+        N.check(lib.pvd_sim_judge_log(self._sim._h, None, 0, ctypes.byref(n), N.stream_handle()), 'pvd_sim_judge_log')
+        if n.value == 0:
+            return []
+        out = np.empty((n.value, 4))
+        N.check(lib.pvd_sim_judge_log(self._sim._h, out.ctypes.data, n.value, ctypes.byref(n), N.stream_handle()),
+                'pvd_sim_judge_log')
+        return [(int(r[0]), float(np.sqrt(r[1] / max(r[3], 1.0))), float(r[2]), int(r[3])) for r in out]
 
     def getpot(self, cds, timeit=False):
         start = time.time() if timeit else None
@@ -303,14 +368,3 @@ class SurrogatePotential(DevicePotential):
         prec = N.PVD_FP64 if sim.precision == 'fp64' else N.PVD_FP32
         self.model.energy_planes(planes.value, cap, ndev.value, cap, self._V, prec, self.fallback)
         return self._V
-
-    def judge(self, sim, step):
-        """RMS / max |V_surrogate - V_analytic| (hartree) on every judge_stride-th walker."""
-        c, _ = sim._get_walkers()
-        sub = c[::self.judge_stride]
-        va = self.fallback.getpot(sub)
-        vs = self.model.energy(sub, self.fallback if self.model.desc.use_range else None)
-        err = vs - va
-        rec = (step, float(np.sqrt(np.mean(err ** 2))), float(np.max(np.abs(err))), len(sub))
-        self.judge_log.append(rec)
-        return rec

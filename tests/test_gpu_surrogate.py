@@ -60,6 +60,31 @@ def test_forward_matches_oracle(hidden):
 
 
 @pytest.mark.parametrize('hidden', [128, 256])
+def test_forward_with_quadratic_baseline_matches_oracle(hidden):
+    """V = ystd y + ymean + q(x) with the baseline coefficients the host fit by least
+    squares; the baseline alone (zero network output scale) is FP32-exact to 1e-6."""
+    import torch
+    import pyvibdmc_amd as pv
+    m = pv.SurrogateModel(['H', 'H', 'O'], hidden=hidden, seed=5)
+    x = water_walkers(3000, 6)
+    e = P.morse_bend(x)
+    m.set_normalisation(x, e)
+    xn = normalised_features(m, x)
+    r = m.fit_baseline(torch.from_numpy(xn.T.copy()).cuda(), torch.from_numpy(e).cuda()).cpu().numpy()
+    c = np.array(m.desc.quad[:10], np.float64)
+    np.testing.assert_allclose(c, O.fit_quad(xn, e), rtol=1e-4, atol=1e-7)
+    base = O.quad_terms(xn) @ c
+    np.testing.assert_allclose(r, e - base, rtol=0, atol=1e-9)
+    y, _ = oracle_net(m).forward(xn)
+    v = m.energy(x)
+    ref = y * m.desc.ystd + m.desc.ymean + base
+    err = np.linalg.norm(v - ref) / np.linalg.norm(ref - ref.mean())
+    assert err < 2e-2, err
+    m.desc.ystd = 0.0
+    np.testing.assert_allclose(m.energy(x), m.desc.ymean + base, rtol=0, atol=1e-6)
+
+
+@pytest.mark.parametrize('hidden', [128, 256])
 def test_gradients_match_oracle(hidden):
     import torch
     import pyvibdmc_amd as pv

@@ -61,3 +61,17 @@ def test_adam_first_step_is_signed_lr():
     opt = O.Adam(p, lr=0.01)
     opt.step(p, {'a': np.array([0.5, -4.0, 0.0], np.float32)})
     np.testing.assert_allclose(p['a'], [0.99, -1.99, 3.0], atol=1e-6)
+
+
+def test_quadratic_baseline_terms_and_fit():
+    """Term order 1, x_i, x_i x_j (i <= j, row order), and the least-squares fit recovers an
+    exactly quadratic function of the features."""
+    x = np.array([[2.0, 3.0, 5.0]])
+    np.testing.assert_array_equal(O.quad_terms(x)[0], [1, 2, 3, 5, 4, 6, 10, 9, 15, 25])
+    rs = np.random.RandomState(1)
+    xn = rs.normal(size=(400, 3))
+    c = rs.normal(size=10)
+    v = O.quad_terms(xn) @ c
+    np.testing.assert_allclose(O.fit_quad(xn, v), c, rtol=1e-10, atol=1e-12)
+    # 15 features (six atoms): 136 terms
+    assert O.quad_terms(rs.normal(size=(5, 15))).shape == (5, 136)
