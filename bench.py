@@ -9,10 +9,12 @@ branching, compaction, vref in the last tile); value = walker-steps/s = sum over
 population entering each step, divided by the wall time between two device syncs.
 
 Also reported: the same case with importance sampling (Gaussian-product psi_T, drift and
-local energy from the trial kernels, Metropolis), the dominant kernel's roofline fraction
-(algorithmic bytes / average launch duration from HIP events, against 8 TB/s) and the CPU
-baseline (the oracle with the reference's process-pool PES manager on the host cores, on a
-bounded sample of the same workload).
+local energy from the trial kernels, Metropolis); the surrogate form (an MLP trained on the
+GPU on 2M geometries an analytic importance-sampled run visits, then driving the same
+importance-sampled DMC); ZPE runs of every form against FP64 runs; the dominant kernel's
+roofline fraction (algorithmic bytes / average launch duration from HIP events, against
+8 TB/s); and the CPU baseline (the oracle with the reference's process-pool PES manager on
+the host cores, on a bounded sample of the same workload).
 """
 import argparse
 import json
@@ -26,6 +28,9 @@ sys.path.insert(0, ROOT)
 import numpy as np  # noqa: E402
 
 HBM_PEAK_GBS = 8000.0
+MFMA_BF16_PEAK_TFLOPS = 2500.0  # dense
+# algorithmic FLOP of one network evaluation: 2 (D H + H H + H H + H), D = 3, H = 128
+MLP_FLOP_PER_WALKER = 2 * (3 * 128 + 128 * 128 + 128 * 128 + 128)
 METRIC = "walker-steps/s at 1M walkers, plain vs IS+surrogate; ZPE err cm^-1 vs FP64"
 # algorithmic HBM bytes per walker-step of the fused discrete step, FP32 triatomic:
 # read 9 coordinates (36 B) + write 9 coordinates per surviving copy (36 B on average,
@@ -38,17 +43,88 @@ def log(*a):
     print(*a, file=sys.stderr, flush=True)
 
 
-def make_sim(n, steps, precision='fp32', imp=False, seed=1234):
+def make_sim(n, steps, precision='fp32', imp=False, seed=1234, pot=None, train_every=None):
     import pyvibdmc_amd as pv
     from pyvibdmc_amd.impsamp import water_trial
     t = np.deg2rad(104.5)
     eq = np.array([[1.81, 0.0, 0.0], [1.81 * np.cos(t), 1.81 * np.sin(t), 0.0], [0.0, 0.0, 0.0]])
     start = eq[None] + np.random.RandomState(seed).normal(0.0, 0.05, size=(n, 3, 3))
     sim = pv.DMC_Sim(num_walkers=n, num_timesteps=steps, atoms=['H', 'H', 'O'], masses=[1837.15, 1837.15, 29156.95],
-                     delta_t=5, potential=pv.MorseBendTriatomic(), start_structures=start,
+                     delta_t=5, potential=pot if pot is not None else pv.MorseBendTriatomic(), start_structures=start,
                      imp_samp=water_trial() if imp else None, precision=precision, rng='philox', seed=seed,
-                     write_files=False, equil_steps=10 ** 9, wfn_every=10 ** 9, chkpt_every=10 ** 9)
+                     write_files=False, equil_steps=10 ** 9, wfn_every=10 ** 9, chkpt_every=10 ** 9,
+                     DEBUG_save_training_every=train_every)
     return sim
+
+
+WN = 4.556335281212229e-6  # hartree per cm^-1
+
+
+def blocking_se(x):
+    """Standard error of the mean of a correlated series: the largest block-average
+    estimate over block sizes 1, 2, 4, ... with at least 16 blocks (Flyvbjerg-Petersen)."""
+    x = np.asarray(x, np.float64)
+    best, b = 0.0, 1
+    while len(x) // b >= 16:
+        m = len(x) // b
+        blocks = x[:m * b].reshape(m, b).mean(axis=1)
+        best = max(best, blocks.std(ddof=1) / np.sqrt(m))
+        b *= 2
+    return best
+
+
+def zpe_of(vref, t=None):
+    """The reference's ZPE estimate after t steps: mean of vref[t//4:t] (pyvibdmc.py:904-908),
+    in cm^-1, with its blocking standard error."""
+    v = np.asarray(vref[:t] if t else vref, np.float64) / WN
+    tail = v[len(v) // 4:]
+    return float(tail.mean()), float(blocking_se(tail))
+
+
+def time_to_accuracy(vref, target, ms_per_step, tol=1.0, every=250):
+    """First step count t (multiple of `every`) at which the running ZPE is within `tol`
+    cm^-1 of `target` with a blocking error below `tol`, and the wall time it costs at the
+    measured step rate."""
+    for t in range(every, len(vref) + 1, every):
+        z, se = zpe_of(vref, t)
+        if abs(z - target) <= tol and se <= tol:
+            return {"steps": t, "seconds": t * ms_per_step * 1e-3}
+    return None
+
+
+def timed_propagate(sim):
+    import torch
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    sim.propagate()
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t0
+    return wall, float(np.sum(sim._pop_vs_tau))
+
+
+def train_surrogate(n, epochs, geoms, seed=11):
+    """The surrogate workflow: geometries visited by an analytic importance-sampled run
+    (training-data hook, every 300 steps), labelled by the analytic surface, then Adam on
+    the GPU.  Returns the model and a summary."""
+    import pyvibdmc_amd as pv
+    snaps = max(1, int(np.ceil(geoms / n)))
+    sim = make_sim(n, 300 * snaps + 1, imp=True, seed=seed, train_every=300)
+    t0 = time.perf_counter()
+    sim.propagate()
+    data = [(c, v) for st, c, v in sim.training if st > 0][:snaps]
+    coords = np.concatenate([c for c, _ in data])[:geoms]
+    pots = np.concatenate([v for _, v in data])[:geoms]
+    t_data = time.perf_counter() - t0
+    del sim
+    m = pv.SurrogateModel(['H', 'H', 'O'], hidden=128)
+    losses = m.fit(coords, pots, epochs=epochs, batch=8192, lr=1e-3)
+    te = coords[::197]
+    err = (m.energy(te) - pv.MorseBendTriatomic().getpot(te)) / WN
+    info = {"geometries": int(len(coords)), "epochs": epochs, "batch": 8192, "lr": 1e-3,
+            "data_seconds": t_data, "train_seconds": m.train_seconds, "final_loss": losses[-1],
+            "rms_err_cm": float(np.sqrt(np.mean(err ** 2))), "max_err_cm": float(np.abs(err).max()),
+            "checked_on": int(len(te))}
+    return m, info
 
 
 def run_timed(sim, warmup, steps, events_steps=0):
@@ -89,9 +165,54 @@ def run_timed(sim, warmup, steps, events_steps=0):
     return ws, wall, float(np.mean(entering)), kern_ms, ev_n
 
 
+def zpe_study(n, T, rates, model):
+    """ZPE runs of T steps at n walkers: plain and importance-sampled DMC in FP32, their
+    FP64 runs (the oracles the FP32 forms are judged against), and the surrogate form.
+    Each ZPE comes with its blocking error; each FP32 form also reports the wall time, at its
+    measured throughput, after which its running ZPE is within 1 cm^-1 of the FP64 run of the
+    same form with an error bar below 1 cm^-1."""
+    import torch
+    import pyvibdmc_amd as pv
+    from pyvibdmc_amd.surrogate import SurrogatePotential
+    runs = [('plain_fp64', 'fp64', False, None), ('plain_fp32', 'fp32', False, None),
+            ('is_fp64', 'fp64', True, None), ('is_fp32', 'fp32', True, None)]
+    if model is not None:
+        runs.append(('is_surrogate_fp32', 'fp32', True, 'surrogate'))
+    out = {"steps": T, "walkers": n, "dt": 5, "estimator": "mean of vref over steps T/4..T (pyvibdmc.py:904-908)"}
+    series = {}
+    for name, prec, imp, kind in runs:
+        pot = SurrogatePotential(model, pv.MorseBendTriatomic(), judge_every=100) if kind else None
+        log(f'[bench] ZPE run {name}: {n} walkers x {T} steps')
+        sim = make_sim(n, T, precision=prec, imp=imp, seed=2024, pot=pot)
+        wall, ws = timed_propagate(sim)
+        series[name] = sim._vref_vs_tau.copy()
+        z, se = zpe_of(series[name])
+        out[name] = {"zpe_cm": z, "se_cm": se, "wall_s": wall, "walker_steps_per_s": ws / wall}
+        log(f'[bench]   {name}: ZPE {z:.2f} +- {se:.2f} cm-1 in {wall:.1f} s')
+        del sim
+        torch.cuda.empty_cache()
+    zp = out['plain_fp64']['zpe_cm']
+    for name, oracle in (('plain_fp32', 'plain_fp64'), ('is_fp32', 'is_fp64'), ('is_surrogate_fp32', 'is_fp64')):
+        if name in out:
+            out[name]['err_vs_fp64_same_form_cm'] = out[name]['zpe_cm'] - out[oracle]['zpe_cm']
+            out[name]['err_vs_plain_fp64_cm'] = out[name]['zpe_cm'] - zp
+    # plain and importance-sampled DMC carry different time-step errors at dt = 5, so each
+    # FP32 form is timed to 1 cm^-1 of the FP64 run of its own form
+    oracle_of = {'plain_fp32': 'plain_fp64', 'is_fp32': 'is_fp64', 'is_surrogate_fp32': 'is_fp64'}
+    tta = {}
+    for name, ms in rates.items():
+        if name in series and ms:
+            tta[name] = time_to_accuracy(series[name], out[oracle_of[name]]['zpe_cm'], ms)
+    out['time_to_1cm_vs_fp64_same_form'] = tta
+    out['dt_bias_is_minus_plain_fp64_cm'] = out['is_fp64']['zpe_cm'] - zp
+    a, b = tta.get('plain_fp32'), tta.get('is_surrogate_fp32')
+    out['speedup_is_surrogate_over_plain'] = (a['seconds'] / b['seconds']) if (a and b) else None
+    return out
+
+
 def cpu_baseline(n):
     from oracle.cpu_ref import cpu_model, time_plain_triatomic
-    steps = 12 if n >= 500_000 else 40
+    steps = 60 if n >= 500_000 else 200
     rate, sec, ws, cores = time_plain_triatomic(num_walkers=n, steps=steps, warmup=1)
     return {"value": rate, "unit": "walker-steps/s", "cores": cores, "kind": "port",
             "sample": f"plain discrete triatomic DMC, FP64 NumPy oracle with a {cores}-process pool PES "
@@ -138,6 +259,10 @@ def main():
     ap.add_argument('--no-variants', action='store_true')
     ap.add_argument('--event-steps', type=int, default=200)
     ap.add_argument('--copy-only', action='store_true', help='only the 1 GiB copy (counter calibration)')
+    ap.add_argument('--no-surrogate', action='store_true')
+    ap.add_argument('--epochs', type=int, default=200)
+    ap.add_argument('--train-geoms', type=int, default=2_000_000)
+    ap.add_argument('--zpe-steps', type=int, default=20000, help='0 skips the ZPE runs')
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU: the DMC walkers of this workload live on a single MI355X "
@@ -174,6 +299,7 @@ def main():
                 "bytes_per_walker_step": bpw,
                 "step_kernels_us": {"k_discrete": kern_ms[0] * 1e3}}
     log(f'[bench] plain: {value:.4g} walker-steps/s, {ms_per_step * 1e3:.2f} us/step, kernels {kern_ms} ms')
+    variants = {}
     del sim
     torch.cuda.empty_cache()
 
@@ -183,7 +309,6 @@ def main():
         roof["frac_of_copy"] = roof["achieved"] / copy_gbs
     log(f'[bench] device copy: {copy_gbs:.0f} GB/s')
 
-    variants = {}
     if not args.no_variants:
         log('[bench] importance-sampled variant')
         sim = make_sim(n, total, precision=args.precision, imp=True)
@@ -198,6 +323,54 @@ def main():
         del sim
         torch.cuda.empty_cache()
 
+    if not args.no_variants and not args.no_surrogate:
+        import pyvibdmc_amd as pv
+        from pyvibdmc_amd.surrogate import SurrogatePotential
+        log(f'[bench] surrogate: sampling {args.train_geoms} geometries, training {args.epochs} epochs')
+        model, tinfo = train_surrogate(n, args.epochs, args.train_geoms)
+        log(f'[bench] surrogate trained in {tinfo["train_seconds"]:.1f} s, rms {tinfo["rms_err_cm"]:.2f} cm-1')
+        spot = SurrogatePotential(model, pv.MorseBendTriatomic(), judge_every=100, judge_stride=100)
+        sim = make_sim(n, W + K + 2, imp=True, pot=spot)
+        ws3, wall3, mean3, _, _ = run_timed(sim, W, K, 0)
+        # the network kernel alone, on the simulation's own walkers and stream
+        from pyvibdmc_amd import _native as N
+        import ctypes
+        planes, stride, ndev = ctypes.c_void_p(), ctypes.c_int64(), ctypes.c_void_p()
+        N.check(N.load().pvd_sim_device_view(sim._h, ctypes.byref(planes), ctypes.byref(stride), ctypes.byref(ndev)))
+        vbuf = torch.empty(stride.value, dtype=torch.float64, device='cuda')
+        for _ in range(3):
+            model.energy_planes(planes.value, stride.value, ndev.value, stride.value, vbuf, N.PVD_FP32, spot.fallback)
+        ea, eb = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        ea.record()
+        for _ in range(20):
+            model.energy_planes(planes.value, stride.value, ndev.value, stride.value, vbuf, N.PVD_FP32, spot.fallback)
+        eb.record()
+        torch.cuda.synchronize()
+        e_ms = ea.elapsed_time(eb) / 20
+        flops = MLP_FLOP_PER_WALKER * mean3
+        variants['is_surrogate'] = {
+            "value": ws3 / wall3, "unit": "walker-steps/s", "ms_per_step": wall3 * 1e3 / K,
+            "config": "importance sampling as above; V from the BF16-MFMA MLP (3 sorted inverse distances -> 128 -> "
+                      "128 -> 128 -> 1, SiLU, FP32 quadratic baseline) with the analytic fallback outside the "
+                      "training range and the analytic judge on every 100th walker every 100 steps",
+            "training": tinfo,
+            "mlp_roofline": {"bound": "mfma", "achieved": flops / (e_ms * 1e-3) / 1e12, "peak": MFMA_BF16_PEAK_TFLOPS,
+                             "unit": "TFLOP/s", "frac": flops / (e_ms * 1e-3) / 1e12 / MFMA_BF16_PEAK_TFLOPS,
+                             "kernel": "k_mlp_energy<float,9,128> + k_mlp_fallback", "avg_launch_us": e_ms * 1e3,
+                             "flop_per_walker": MLP_FLOP_PER_WALKER},
+            "judge_rms_cm": [r[1] / WN for r in spot.judge_log][-3:]}
+        log(f'[bench] IS+surrogate: {ws3 / wall3:.4g} walker-steps/s, network {e_ms * 1e3:.1f} us/launch')
+        del sim, vbuf
+        torch.cuda.empty_cache()
+    else:
+        model = None
+
+    zpe = None
+    if not args.no_variants and args.zpe_steps > 0:
+        rates = {'plain_fp32': ms_per_step, 'is_fp32': (variants.get('is') or {}).get('ms_per_step'),
+                 'is_surrogate_fp32': (variants.get('is_surrogate') or {}).get('ms_per_step')}
+        zpe = zpe_study(n, args.zpe_steps, rates, model)
+
     cpu = None
     if not args.no_cpu:
         log('[bench] CPU baseline sample')
@@ -211,7 +384,7 @@ def main():
                                   f"birth/death, {args.precision} coordinates with FP64 weights/reductions, "
                                   f"Philox draws",
                       "walkers": n, "mean_population": mean_n, "parallelism": "single GPU"},
-           "roofline": roof, "cpu_baseline": cpu, "variants": variants}
+           "roofline": roof, "cpu_baseline": cpu, "variants": variants, "zpe": zpe}
     print(json.dumps(out), flush=True)
 
 
