@@ -284,8 +284,27 @@ class SurrogateModel:
             if max_seconds and time.time() - start > max_seconds:
                 break
         self.history += losses
+        self.calibrate_offset(coords, energies)
         self.train_seconds = time.time() - start
         return losses
+
+    def calibrate_offset(self, coords, energies, samples=262144):
+        """Zero the mean error of the deployed network (the inference kernel, BF16 rounding
+        included) on the training geometries by shifting the output offset.  A DMC energy
+        estimate sees the surrogate's mean error over the sampled distribution directly, and
+        the training data are that distribution."""
+        n = len(energies)
+        step = max(1, n // samples)
+        c = np.asarray(coords)[::step]
+        e = np.asarray(energies, np.float64)[::step]
+        keep = self.desc.use_range
+        self.desc.use_range = 0
+        try:
+            d = float(np.mean(self.energy(c) - e))
+        finally:
+            self.desc.use_range = keep
+        self.desc.ymean = float(self.desc.ymean - d)
+        return d
 
     # -- inference ----------------------------------------------------------------------
     def energy_planes(self, planes_ptr, stride, n_dev_ptr, n_max, out, precision, fallback=None):
