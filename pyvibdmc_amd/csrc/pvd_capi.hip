@@ -255,6 +255,7 @@ struct pvd_sim {
   unsigned long long* kmask;
   Ctrl* ctrl;
   uint64_t* flags;
+  uint64_t* gflags;
   TileRec* rec;
   uint64_t* drec;
   unsigned* accs;
@@ -280,6 +281,10 @@ struct pvd_sim {
   unsigned epoch_next;
   int status_seen;
   int use_graphs;
+  int cus;                      // compute units of the device
+  const void* slot_fn[16];      // resident-workgroup counts of the discrete kernels used so far
+  int slot_val[16];
+  int n_slot;
   hipGraphExec_t graph[2];  // replayed step graphs, without / with descendant tracking
   // surrogate potential (external-potential simulations only)
   int surr_on;
@@ -318,6 +323,7 @@ StepArgs<R> make_args(pvd_sim* s) {
   A.kmask = s->kmask;
   A.ctrl = s->ctrl;
   A.flags = s->flags;
+  A.gflags = s->gflags;
   A.rec = s->rec;
   A.drec = s->drec;
   A.accs = s->accs;
@@ -406,8 +412,31 @@ struct Launch {
   dim3 grid, block;
 };
 
+// Resident workgroups of a discrete step kernel on this device (the tiling unit), cached.
+int disc_slots(pvd_sim* s, const void* fn) {
+  for (int i = 0; i < s->n_slot; ++i)
+    if (s->slot_fn[i] == fn) return s->slot_val[i];
+  int per_cu = 0;
+  if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, fn, TILE, 0) != hipSuccess || per_cu < 1) per_cu = 1;
+  const int v = per_cu * std::max(1, s->cus);
+  if (s->n_slot < 16) {
+    s->slot_fn[s->n_slot] = fn;
+    s->slot_val[s->n_slot] = v;
+    ++s->n_slot;
+  }
+  return v;
+}
+
+// Grid of a discrete step kernel: enough workgroups for the tiling of any population up to
+// the capacity (discrete_tiles); workgroups past the tile count exit at once.
+unsigned disc_grid(const pvd_sim* s, int slots, int W) {
+  const long long per_round = (long long)slots * TILE * W;
+  const long long rounds = (s->cap + per_round - 1) / per_round;
+  return (unsigned)std::max(1ll, rounds * slots);
+}
+
 template <typename R, bool INJ>
-int step_kernels(pvd_sim* s, Launch* L, int& nl) {
+int step_kernels(pvd_sim* s, StepArgs<R>& A, Launch* L, int& nl) {
   const unsigned g = nblk(s->cap, TILE);
   nl = 0;
   const int pk = s->cfg.pes.kind;
@@ -416,12 +445,15 @@ int step_kernels(pvd_sim* s, Launch* L, int& nl) {
 #define PVD_ADD(...) PVD_STEP_SWITCH(s, add((const void*)(__VA_ARGS__), g, TILE))
   if (s->is) PVD_ADD(k_is_move<R, NP, INJ, MS>);
   if (!s->cont) {
+    auto add_disc = [&](const void* fn, int W) {
+      A.slots = disc_slots(s, fn);
+      add(fn, disc_grid(s, A.slots, W), TILE);
+    };
     if (s->is) {
-      PVD_STEP_SWITCH(s, add((const void*)k_discrete<R, NP, INJ, true, false, MS>,
-                             nblk(s->cap, TILE * disc_wpt<R, NP, false, true>()), TILE));
+      PVD_STEP_SWITCH(s, add_disc((const void*)k_discrete<R, NP, INJ, true, false, MS>, disc_wpt<R, NP, false, true>()));
     } else {
-      PVD_STEP_SWITCH(s, add((const void*)k_discrete<R, NP, INJ, false, false, (MS & MS_PES_WATER)>,
-                             nblk(s->cap, TILE * disc_wpt<R, NP, false>()), TILE));
+      PVD_STEP_SWITCH(s, add_disc((const void*)k_discrete<R, NP, INJ, false, false, (MS & MS_PES_WATER)>,
+                                  disc_wpt<R, NP, false>()));
     }
   } else {
     if (s->is) {
@@ -439,7 +471,7 @@ template <typename R, bool INJ>
 int launch_fused_step(pvd_sim* s, StepArgs<R>& A, hipStream_t st) {
   Launch L[8];
   int nl = 0;
-  int rc = step_kernels<R, INJ>(s, L, nl);
+  int rc = step_kernels<R, INJ>(s, A, L, nl);
   if (rc) return rc;
   void* args[] = {&A};
   for (int i = 0; i < nl; ++i) HIPCHK(hipLaunchKernel(L[i].fn, L[i].grid, L[i].block, args, 0, st));
@@ -464,7 +496,7 @@ int build_graph(pvd_sim* s, hipGraphExec_t* out) {
     StepArgs<R> A = make_args<R>(s);
     Launch L[8];
     int nl = 0;
-    rc = step_kernels<R, false>(s, L, nl);
+    rc = step_kernels<R, false>(s, A, L, nl);
     for (int i = 0; i < nl && rc == 0; ++i) {
       hipKernelNodeParams p;
       std::memset(&p, 0, sizeof(p));
@@ -571,7 +603,8 @@ template <typename R>
 int run_steps_timed(pvd_sim* s, int n_steps, hipStream_t st, float* avg_ms, int32_t* n_kernels) {
   Launch L[8];
   int nl = 0;
-  int rc = step_kernels<R, false>(s, L, nl);
+  StepArgs<R> A0 = make_args<R>(s);
+  int rc = step_kernels<R, false>(s, A0, L, nl);
   if (rc) return rc;
   std::vector<hipEvent_t> ev((size_t)2 * nl * n_steps, nullptr);
   auto release = [&]() {
@@ -586,6 +619,7 @@ int run_steps_timed(pvd_sim* s, int n_steps, hipStream_t st, float* avg_ms, int3
   }
   for (int k = 0; k < n_steps; ++k) {
     StepArgs<R> A = make_args<R>(s);
+    A.slots = A0.slots;
     void* args[] = {&A};
     for (int i = 0; i < nl; ++i) {
       // start / stop events recorded by the dispatch itself (kernel execution only)
@@ -642,15 +676,14 @@ int do_branch_ext(pvd_sim* s, const double* v, const double* ub, hipStream_t st)
   StepArgs<R> A = make_args<R>(s);
   A.V_ext = v;
   A.inj_ubr = ub;
-  const unsigned g = s->cont ? nblk(s->cap, TILE) : nblk(s->cap, TILE * disc_wpt<R, 1, true>());
+  unsigned g = nblk(s->cap, TILE);
   if (!s->cont) {
-    if (ub) {
-      if (s->is) hipLaunchKernelGGL((k_discrete<R, 1, true, true, true>), dim3(g), dim3(TILE), 0, st, A);
-      else hipLaunchKernelGGL((k_discrete<R, 1, true, false, true>), dim3(g), dim3(TILE), 0, st, A);
-    } else {
-      if (s->is) hipLaunchKernelGGL((k_discrete<R, 1, false, true, true>), dim3(g), dim3(TILE), 0, st, A);
-      else hipLaunchKernelGGL((k_discrete<R, 1, false, false, true>), dim3(g), dim3(TILE), 0, st, A);
-    }
+    const void* fn = ub ? (s->is ? (const void*)k_discrete<R, 1, true, true, true> : (const void*)k_discrete<R, 1, true, false, true>)
+                        : (s->is ? (const void*)k_discrete<R, 1, false, true, true> : (const void*)k_discrete<R, 1, false, false, true>);
+    A.slots = disc_slots(s, fn);
+    g = disc_grid(s, A.slots, disc_wpt<R, 1, true>());
+    void* args[] = {&A};
+    HIPCHK(hipLaunchKernel(fn, dim3(g), dim3(TILE), args, 0, st));
   } else {
     if (s->is) hipLaunchKernelGGL((k_cont<R, 1, false, true, true>), dim3(g), dim3(TILE), 0, st, A);
     else hipLaunchKernelGGL((k_cont<R, 1, false, false, true>), dim3(g), dim3(TILE), 0, st, A);
@@ -727,7 +760,7 @@ void free_sim(pvd_sim* s) {
   for (int i = 0; i < 2; ++i)
     if (s->graph[i]) (void)hipGraphExecDestroy(s->graph[i]);
   void* ptrs[] = {s->coords[0], s->coords[1], s->who[0], s->who[1], s->w, s->E, s->ke, s->kmask,
-                  s->ctrl, s->flags, s->rec, s->drec, s->accs, s->tr_vref, s->tr_pop, s->tr_dteff, s->tr_births,
+                  s->ctrl, s->flags, s->gflags, s->rec, s->drec, s->accs, s->tr_vref, s->tr_pop, s->tr_dteff, s->tr_births,
                   s->tr_deaths, s->pend, s->fin, s->klist, s->killtag, s->rstage, s->stage, s->parent, s->parent_w, s->desc,
                   s->cnt, s->dacc, s->vsur, s->jlog};
   for (void* p : ptrs)
@@ -780,7 +813,16 @@ int pvd_sim_create(const pvd_config* cfg, pvd_sim** out) {
   }
   s->cap = cap;
   s->tmax = cfg->num_timesteps > 0 ? cfg->num_timesteps : 1;
-  const long long ntiles = cap / TILE;
+  {
+    int dev = 0, cus = 0;
+    if (hipGetDevice(&dev) == hipSuccess && hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) ==
+                                                 hipSuccess)
+      s->cus = cus;
+    if (s->cus < 1) s->cus = 1;
+  }
+  // tiles of the discrete kernels: at most cap / TILE plus one round of resident workgroups
+  // (at most 8 per CU) - see discrete_tiles and disc_grid
+  const long long ntiles = cap / TILE + 8ll * s->cus;
   const size_t es = esize(s);
 #define ALLOC(ptr, bytes)                                    \
   do {                                                       \
@@ -800,6 +842,7 @@ int pvd_sim_create(const pvd_config* cfg, pvd_sim** out) {
   ALLOC(s->kmask, sizeof(unsigned long long) * ntiles * (TILE / PVD_WAVE));
   ALLOC(s->ctrl, sizeof(Ctrl));
   ALLOC(s->flags, sizeof(uint64_t) * ntiles);
+  ALLOC(s->gflags, sizeof(uint64_t) * (ntiles / LB_GROUP + 1));
   ALLOC(s->rec, sizeof(TileRec) * ntiles);
   ALLOC(s->drec, sizeof(uint64_t) * 3 * ntiles);
   ALLOC(s->accs, sizeof(unsigned) * 2 * ACC_SLOTS * ACC_STRIDE);
@@ -818,6 +861,7 @@ int pvd_sim_create(const pvd_config* cfg, pvd_sim** out) {
   ALLOC(s->stage, sizeof(double) * np * cap);
 #undef ALLOC
   hipError_t e = hipMemset(s->flags, 0, sizeof(uint64_t) * ntiles);
+  if (e == hipSuccess) e = hipMemset(s->gflags, 0, sizeof(uint64_t) * (ntiles / LB_GROUP + 1));
   if (e == hipSuccess) e = hipMemset(s->drec, 0, sizeof(uint64_t) * 3 * ntiles);
   if (e == hipSuccess) e = hipMemset(s->accs, 0, sizeof(unsigned) * 2 * ACC_SLOTS * ACC_STRIDE);
   if (e == hipSuccess) e = hipMemset(s->ctrl, 0, sizeof(Ctrl));

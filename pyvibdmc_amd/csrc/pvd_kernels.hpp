@@ -3,18 +3,18 @@
 // Walkers live in HBM as structure-of-arrays planes: component p of walker slot i is at
 // coords[p * cap + i] (p = atom*d + c), double-buffered for the discrete branch.  One
 // step of the discrete algorithm (reference pyvibdmc.py:540-547, 786-793, 391-431,
-// 651-661) is ONE launch of k_discrete: each 256-walker tile moves its walkers (Philox
-// + Box-Muller, or injected draws), evaluates the PES in registers, forms the
-// branching weight and copy count, and writes its survivors to their final slots in
-// np.repeat order.  The output offset of a tile comes from a single-pass decoupled
-// look-back over per-tile flags.  A flag is one 64-bit word (epoch | status | prefix)
-// written and read with relaxed agent-scope atomics, so publishing it needs no fence:
-// on CDNA an agent-scope release writes back the whole L2 of the XCD, which per tile
-// would cost more than the step itself.  The per-tile partial sums are plain stores that
-// a one-workgroup finalise launch reduces in a fixed order (the kernel boundary makes
-// them visible), writing vref and the population of the next step.  All control values
-// are double-buffered by step parity `par`, which is a launch argument, so a launch never
-// reads a value that the same launch writes.
+// 651-661) is ONE launch of k_discrete: each tile (one workgroup, 4 walkers per thread)
+// moves its walkers (Philox + Box-Muller, or injected draws), evaluates the PES in
+// registers, forms the branching weight and copy count, and writes its survivors to their
+// final slots in np.repeat order.  The output offset of a tile is the sum of the published
+// counts of its predecessors, read in two levels (tiles within a 64-tile group, then group
+// totals).  Counts are 64-bit words (epoch | value) written and read with relaxed
+// agent-scope atomics, so publishing needs no fence: on CDNA an agent-scope release writes
+// back the whole L2 of the XCD, which per tile would cost more than the step itself.  Each
+// tile also publishes a record of its partial sums, which the last tile reduces in a fixed
+// order into vref and the population of the next step.  All control values are
+// double-buffered by step parity `par`, which is a launch argument, so a launch never reads
+// a value that the same launch writes.
 #pragma once
 #include "pvd_models.hpp"
 
@@ -84,6 +84,7 @@ struct StepArgs {
   unsigned long long* kmask;  // continuous: per-wave kill masks
   Ctrl* ctrl;
   uint64_t* flags;
+  uint64_t* gflags;  // discrete: per 64-tile group totals
   TileRec* rec;
   uint64_t* drec;      // discrete: 3 coherent words per tile (sum c*E | births,count | epoch,bad,deaths)
   unsigned* accs;      // importance sampling: accepted-move counters (see acc_slot)
@@ -109,6 +110,7 @@ struct StepArgs {
   double dt, alpha, n0, pop_lo, pop_hi, thresh_lo;
   Rng rng;
   int par;
+  int slots;   // discrete: resident workgroups of the step kernel (tiling unit)
   int branch;
   int ms, ds;  // atoms, dims
   int is_;     // importance sampling on
@@ -364,59 +366,80 @@ PVD_DEV R is_move(const StepArgs<R>& A, const R* disp, double u, R* x, bool& acc
   return local_kin<R, NP, MS>(A, acc ? ty.d2 : tx.d2);
 }
 
-// pack / unpack a look-back flag: epoch (32) | status (2) | value (30)
-PVD_DEV uint64_t pack_flag(unsigned epoch, unsigned st, unsigned v) {
-  return ((uint64_t)epoch << 32) | ((uint64_t)st << 30) | (uint64_t)(v & 0x3FFFFFFFu);
-}
-enum : unsigned { FL_AGG = 1u, FL_INC = 2u };
+// Exclusive prefix of tile `tile` from published counts, by all lanes of wave 0:
+//   flags[t]  = epoch | count of tile t                       (published by every tile)
+//   gflags[g] = epoch | total of the LB_GROUP tiles of group g (published by its last tile)
+// prefix(t) = sum of gflags over the groups before t's + sum of flags over all of t's
+// predecessors in its group, read directly (4 loads per lane in flight per pass).  Every
+// value is final once published, so a tile waits only for its predecessors to finish their
+// own work: one coherent round trip for up to LB_GROUP tiles (1M walkers), two beyond.
+// Predecessors are resident or finished (tiles are dispatched in index order), so every wait
+// ends; the spin is still bounded and reports ST_SPIN if it expires.
+constexpr int LB_GROUP = 1024;
 
-// Exclusive prefix of tile `tile` from the flags of its predecessors.  Called by all
-// lanes of wave 0.  Predecessors obtained their tickets earlier and are resident, so
-// every wait ends; the spin is still bounded and reports ST_SPIN if it ever expires.
-PVD_DEV int look_back(uint64_t* flags, int tile, unsigned epoch, Ctrl* C, bool& spun_out) {
-  const int l = lane_id();
-  int prefix = 0;
-  int j = tile - 1;
-  spun_out = false;
-  while (true) {
-    const int t = j - l;
-    uint64_t f = 0;
-    unsigned st = FL_INC, val = 0;
-    if (t >= 0) {
-      long long spins = 0;
-      while (true) {
-        f = ld_relaxed(&flags[t]);
-        if ((unsigned)(f >> 32) == epoch && ((f >> 30) & 3u) != 0u) break;
-        if (++spins > (1ll << 26)) {
-          spun_out = true;
-          break;
-        }
-        __builtin_amdgcn_s_sleep(1);
-      }
-      st = (unsigned)((f >> 30) & 3u);
-      val = (unsigned)(f & 0x3FFFFFFFu);
-    }
-    const bool anyspin = __any(spun_out);
-    if (anyspin) {
-      spun_out = true;
-      return 0;
-    }
-    const unsigned long long inc_mask = __ballot(st == FL_INC);
-    if (inc_mask) {
-      const int k = __ffsll((long long)inc_mask) - 1;  // first (closest) inclusive flag
-      const int contrib = (l <= k) ? (int)val : 0;
-      prefix += wave_sum(contrib);
-      return prefix;
-    }
-    prefix += wave_sum((int)val);
-    j -= PVD_WAVE;
+PVD_DEV bool spin_epoch(const uint64_t* p, unsigned epoch, uint64_t& f) {
+  long long spins = 0;
+  while ((unsigned)(f >> 32) != epoch) {
+    if (++spins > (1ll << 26)) return false;
+    __builtin_amdgcn_s_sleep(1);
+    f = ld_relaxed(p);
   }
+  return true;
+}
+
+// sum of the epoch-tagged values base[0 .. cnt), by all lanes of a wave
+PVD_DEV unsigned sum_published(const uint64_t* base, int cnt, unsigned epoch, bool& ok) {
+  const int l = lane_id();
+  unsigned v = 0;
+  for (int t0 = 0; t0 < cnt; t0 += 4 * PVD_WAVE) {
+    uint64_t f[4];
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+      const int t = t0 + q * PVD_WAVE + l;
+      f[q] = t < cnt ? ld_relaxed(base + t) : ((uint64_t)epoch << 32);
+    }
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+      const int t = t0 + q * PVD_WAVE + l;
+      if (t < cnt) {
+        if (ok) ok = spin_epoch(base + t, epoch, f[q]);
+        v += (unsigned)f[q];
+      }
+    }
+  }
+  return (unsigned)wave_sum((int)v);
+}
+
+PVD_DEV int grouped_prefix(uint64_t* flags, uint64_t* gflags, int tile, unsigned epoch, unsigned total,
+                           bool& spun_out) {
+  const int l = lane_id();
+  const int g = tile / LB_GROUP, j = tile % LB_GROUP;
+  bool ok = true;
+  const unsigned ip = sum_published(flags + (size_t)g * LB_GROUP, j, epoch, ok);
+  if (j == LB_GROUP - 1 && l == 0) st_relaxed(gflags + g, ((uint64_t)epoch << 32) | (uint64_t)(ip + total));
+  const unsigned gp = g > 0 ? sum_published(gflags, g, epoch, ok) : 0u;
+  spun_out = __any(!ok);
+  return (int)(gp + ip);
 }
 
 // ----------------------------------------------------------------------------- discrete
 // Walkers per thread of the discrete step: a tile is TILE * W walkers, walker
 // (k, thread) of tile b is slot b*TILE*W + k*TILE + thread (coalesced per k).  Larger tiles
-// mean fewer look-back flags and shorter look-back chains.
+// mean fewer published counts and cheaper per-tile scans.
+// Tiling of n walkers: the number of tiles is a multiple of the resident workgroup count
+// `slots` (or n / TILE when that is smaller), and all tiles are equally large (at most TW).
+// Every CU then holds the same number of equally long tiles, so no tile's output offset
+// waits on a predecessor from a more crowded CU.
+PVD_DEV void discrete_tiles(int n, int slots, int TW, int& ntiles, int& tw) {
+  const long long per_round = (long long)(slots > 0 ? slots : 1) * TW;
+  long long nt = ((n + per_round - 1) / per_round) * (long long)(slots > 0 ? slots : 1);
+  const long long nmin = ((long long)n + TILE - 1) / TILE;
+  if (nt > nmin) nt = nmin;
+  if (nt < 1) nt = 1;
+  ntiles = (int)nt;
+  tw = (int)(((long long)n + nt - 1) / nt);
+}
+
 template <typename R, int NP, bool EXT, bool IS = false>
 constexpr int disc_wpt() {
   return EXT ? 4 : ((int)sizeof(R) * NP <= 36 ? 4 : ((int)sizeof(R) * NP <= 72 ? 2 : 1));
@@ -424,10 +447,10 @@ constexpr int disc_wpt() {
 
 // Finalise a discrete step inside its last tile: the records of all tiles, each read once
 // its epoch word shows the current step (a tile writes that word after its other words
-// have completed, and before its look-back flag), a fixed-order reduction, the population
-// guards (pyvibdmc.py:397-400, 409-413), calc_vref (pyvibdmc.py:655-661) and the per-step
-// record (pyvibdmc.py:674-682).  `expect` is the population from the look-back, checked
-// against the records.  Called by every thread of the tile.
+// have completed, and before its count), a fixed-order reduction, the population guards
+// (pyvibdmc.py:397-400, 409-413), calc_vref (pyvibdmc.py:655-661) and the per-step record
+// (pyvibdmc.py:674-682).  `expect` is the population from the prefix read, checked against
+// the records.  Called by every thread of the tile.
 template <typename R, bool IS>
 PVD_DEV void finalize_discrete(const StepArgs<R>& A, Ctrl* C, int par, int n, int ntiles, unsigned epoch,
                                unsigned long long step, double dte, int expect) {
@@ -535,7 +558,8 @@ __global__ __launch_bounds__(TILE) void k_discrete(StepArgs<R> A) {
   Ctrl* C = A.ctrl;
   const int tile = blockIdx.x;
   const int n = C->n[par];
-  const int ntiles = (n + TW - 1) / TW;
+  int ntiles, tw;
+  discrete_tiles(n, A.slots, TW, ntiles, tw);
   if (C->status != ST_OK || tile >= ntiles) return;
   const unsigned long long step = C->step[par];
   const unsigned epoch = C->epoch[par];
@@ -549,8 +573,9 @@ __global__ __launch_bounds__(TILE) void k_discrete(StepArgs<R> A) {
   int births = 0, deaths = 0, bad = 0;
 #pragma unroll
   for (int k = 0; k < W; ++k) {
-    const long long i = (long long)tile * TW + k * TILE + threadIdx.x;
-    const bool valid = i < n;
+    const int li = k * TILE + threadIdx.x;
+    const long long i = (long long)tile * tw + li;
+    const bool valid = li < tw && i < n;
     double E = 0.0, um = 0.0, ub = 0.0;
     if (valid) {
       if constexpr (EXT) {
@@ -648,7 +673,7 @@ __global__ __launch_bounds__(TILE) void k_discrete(StepArgs<R> A) {
   }
 
   // publish the tile record (the epoch word last, after the others have completed), then
-  // the look-back flag, then look back for the exclusive prefix
+  // the tile's count, then read the exclusive prefix
   if (threadIdx.x == 0) {
     uint64_t* dr = A.drec + 3 * (size_t)tile;
     st_relaxed(dr + 0, (uint64_t)__double_as_longlong(sce));
@@ -656,18 +681,13 @@ __global__ __launch_bounds__(TILE) void k_discrete(StepArgs<R> A) {
     store_complete();
     st_relaxed(dr + 2, ((uint64_t)epoch << 32) | ((uint64_t)(uint32_t)bad << 16) | (uint64_t)(uint32_t)deaths);
     store_complete();
-    st_relaxed(&A.flags[tile], pack_flag(epoch, tile == 0 ? FL_INC : FL_AGG, (unsigned)total));
+    st_relaxed(&A.flags[tile], ((uint64_t)epoch << 32) | (uint64_t)(unsigned)total);
   }
   if (threadIdx.x < PVD_WAVE) {
-    int prefix = 0;
     bool spun = false;
-    if (tile > 0) prefix = look_back(A.flags, tile, epoch, C, spun);
+    const int prefix = grouped_prefix(A.flags, A.gflags, tile, epoch, (unsigned)total, spun);
     if (threadIdx.x == 0) {
-      if (spun) {
-        st_release_i(&C->status, ST_SPIN);
-      } else if (tile > 0) {
-        st_relaxed(&A.flags[tile], pack_flag(epoch, FL_INC, (unsigned)(prefix + total)));
-      }
+      if (spun) st_release_i(&C->status, ST_SPIN);
       s_prefix = prefix;
       s_status = spun ? 1 : 0;
     }
@@ -678,8 +698,9 @@ __global__ __launch_bounds__(TILE) void k_discrete(StepArgs<R> A) {
   // survivors to their np.repeat positions
 #pragma unroll
   for (int k = 0; k < W; ++k) {
-    const long long i = (long long)tile * TW + k * TILE + threadIdx.x;
-    if (i < n && c[k] > 0) {
+    const int li = k * TILE + threadIdx.x;
+    const long long i = (long long)tile * tw + li;
+    if (li < tw && i < n && c[k] > 0) {
       const long long base = (long long)s_prefix + excl[k];
       for (int q = 0; q < c[k]; ++q) {
         const long long dst = base + q;
@@ -698,7 +719,8 @@ __global__ __launch_bounds__(TILE) void k_discrete(StepArgs<R> A) {
       }
     }
   }
-  // the last tile finishes its look-back after every other tile has published its record
+  // the last tile has its prefix only after every other tile has published its count, and a
+  // tile publishes its record before its count
   if (tile == ntiles - 1) finalize_discrete<R, IS>(A, C, par, n, ntiles, epoch, step, dte, s_prefix + total);
 }
 
