@@ -369,12 +369,11 @@ StepArgs<R> make_args(pvd_sim* s) {
 int model_spec(const pvd_config& c, int np) {
   if (np != 9 || c.n_atoms != 3 || c.n_dim != 3) return MS_ANY;
   const pvd_pes_desc& p = c.pes;
-  if (!(p.kind == PES_MORSE_BEND && p.idx[0] == 0 && p.idx[1] == 1 && p.idx[2] == 2)) return MS_ANY;
   const pvd_trial_desc& t = c.trial;
-  if (t.kind == TRIAL_NONE) return MS_PES_WATER;
+  const bool pw = p.kind == PES_MORSE_BEND && p.idx[0] == 0 && p.idx[1] == 1 && p.idx[2] == 2;
   const bool tw = t.kind == TRIAL_GAUSS && t.nb == 2 && t.na == 1 && t.bi[0] == 0 && t.bj[0] == 2 &&
                   t.bi[1] == 1 && t.bj[1] == 2 && t.ai[0] == 0 && t.aj[0] == 2 && t.ak[0] == 1;
-  return tw ? (MS_PES_WATER | MS_TRIAL_WATER) : MS_PES_WATER;
+  return (pw ? MS_PES_WATER : MS_ANY) | (tw ? MS_TRIAL_WATER : MS_ANY);
 }
 
 // ---- launch helpers, instantiated over the component count ----------------------------
@@ -392,6 +391,9 @@ int model_spec(const pvd_config& c, int np) {
   PVD_NP_SWITCH((sim)->np, if constexpr (NP == 9) {                             \
     if ((sim)->mspec == (MS_PES_WATER | MS_TRIAL_WATER)) {                      \
       constexpr int MS = MS_PES_WATER | MS_TRIAL_WATER;                         \
+      __VA_ARGS__;                                                              \
+    } else if ((sim)->mspec == MS_TRIAL_WATER) {                                \
+      constexpr int MS = MS_TRIAL_WATER;                                        \
       __VA_ARGS__;                                                              \
     } else if ((sim)->mspec == MS_PES_WATER) {                                  \
       constexpr int MS = MS_PES_WATER;                                          \
@@ -659,9 +661,9 @@ int do_move(pvd_sim* s, const double* disp, const double* um, hipStream_t st) {
   const unsigned g = nblk(s->cap, TILE);
   if (s->is) {
     if (disp) {
-      PVD_NP_SWITCH(s->np, hipLaunchKernelGGL((k_move<R, NP, true, true>), dim3(g), dim3(TILE), 0, st, A));
+      PVD_STEP_SWITCH(s, hipLaunchKernelGGL((k_move<R, NP, true, true, (MS & MS_TRIAL_WATER)>), dim3(g), dim3(TILE), 0, st, A));
     } else {
-      PVD_NP_SWITCH(s->np, hipLaunchKernelGGL((k_move<R, NP, false, true>), dim3(g), dim3(TILE), 0, st, A));
+      PVD_STEP_SWITCH(s, hipLaunchKernelGGL((k_move<R, NP, false, true, (MS & MS_TRIAL_WATER)>), dim3(g), dim3(TILE), 0, st, A));
     }
   } else {
     if (disp) hipLaunchKernelGGL((k_move<R, 1, true, false>), dim3(g), dim3(TILE), 0, st, A);

@@ -292,6 +292,36 @@ PVD_DEV R local_kin(const StepArgs<R>& A, const R* sec) {
 // kinetic part at that position; `acc` tells whether the proposal was accepted.
 template <typename R, int NP, bool INJ, int MS = MS_ANY>
 PVD_DEV R is_move(const StepArgs<R>& A, const R* disp, double u, R* x, bool& acc) {
+  if constexpr ((MS & MS_TRIAL_WATER) != 0 && NP == 9 && sizeof(R) == 4) {
+    // FP32 water trial: gradients and the closed-form weighted Laplacian only
+    const R im_h = (R)A.inv_m[0], im_o = (R)A.inv_m[6];
+    const R dt = (R)A.dt;
+    R lx, ly, lapx, lapy, gx[9], gy[9], y[9];
+    trial_water_fast<R>(A.trial, x, im_h, im_o, lx, gx, lapx);
+#pragma unroll
+    for (int p = 0; p < 9; ++p) y[p] = (x[p] + disp[p]) + ((R)A.inv_m[p] * gx[p]) * dt;
+    trial_water_fast<R>(A.trial, y, im_h, im_o, ly, gy, lapy);
+    R la = R(2) * (ly - lx);
+#pragma unroll
+    for (int p = 0; p < 9; ++p) {
+      const R a1 = (x[p] - y[p]) - ((R)A.inv_m[p] * gy[p]) * dt;
+      const R a2 = (y[p] - x[p]) - ((R)A.inv_m[p] * gx[p]) * dt;
+      la += (a2 * a2 - a1 * a1) * (R)A.h2s[p];
+    }
+    acc = (double)pvd_exp<R>(la) > u;
+    R gs = R(0);
+    if (acc) {
+#pragma unroll
+      for (int p = 0; p < 9; ++p) {
+        x[p] = y[p];
+        gs += (R)A.inv_m[p] * (gy[p] * gy[p]);
+      }
+      return R(-0.5) * (gs + lapy);
+    }
+#pragma unroll
+    for (int p = 0; p < 9; ++p) gs += (R)A.inv_m[p] * (gx[p] * gx[p]);
+    return R(-0.5) * (gs + lapx);
+  }
   TrialOut<R, NP> tx, ty;
   trial_eval_s<R, NP, MS>(A.trial, x, tx);
   R y[NP], Dx[NP], Dy[NP];
@@ -762,7 +792,7 @@ __global__ __launch_bounds__(TILE) void k_is_move(StepArgs<R> A) {
 // Moves walkers in place (plain or importance-sampled) so that an outside PES can be
 // evaluated on the moved walkers.  IS writes the local kinetic energy per walker and
 // counts acceptances.
-template <typename R, int NP, bool INJ, bool IS>
+template <typename R, int NP, bool INJ, bool IS, int MS = MS_ANY>
 __global__ __launch_bounds__(TILE) void k_move(StepArgs<R> A) {
   __shared__ int s_i[TILE / PVD_WAVE];
   const int par = A.par;
@@ -781,7 +811,7 @@ __global__ __launch_bounds__(TILE) void k_move(StepArgs<R> A) {
       load_walker<R, NP>(A, i, x);
       draw_step<R, NP, INJ, true>(A, i, step, disp, um, ub, false);
       bool acc;
-      const R ke = is_move<R, NP, INJ>(A, disp, um, x, acc);
+      const R ke = is_move<R, NP, INJ, MS>(A, disp, um, x, acc);
       a = acc ? 1 : 0;
       A.ke[i] = (double)ke;
 #pragma unroll

@@ -453,6 +453,57 @@ PVD_DEV R pes_eval_s(const PesParams& P, const R* x) {
   }
 }
 
+// The [H, H, O] Gaussian-product trial (bonds 0-2 and 1-2, angle 0-2-1) for importance
+// sampling: ln psi, g = grad ln psi (9 components) and the mass-weighted Laplacian
+// lap = sum_a (1/m_a) lap_a ln psi in closed form, so the local kinetic energy is
+// -1/2 (sum_p (1/m_p) g_p^2 + lap) without per-component second derivatives:
+//   lap_a ln psi = -sum_k alpha_k (|grad_a q_k|^2 + (q_k - q0_k) lap_a q_k),
+//   bond r:  |grad r|^2 = 1, lap r = 2/r at both atoms;
+//   angle theta (vertex j, arms u = x_i - x_j, v = x_k - x_j): |grad_i|^2 = 1/ru^2,
+//   |grad_k|^2 = 1/rv^2, |grad_j|^2 = 1/ru^2 + 1/rv^2 - 2 cos/(ru rv); lap_i = cot/ru^2,
+//   lap_k = cot/rv^2, lap_j = cot (1/ru^2 + 1/rv^2) - 2/(ru rv sin).
+// im_h, im_o: 1/m of the hydrogens and of the oxygen.
+template <typename R>
+PVD_DEV void trial_water_fast(const TrialParams& T, const R* x, R im_h, R im_o, R& lnpsi, R* g, R& lap) {
+  R u[3], v[3];
+#pragma unroll
+  for (int c = 0; c < 3; ++c) {
+    u[c] = x[c] - x[6 + c];
+    v[c] = x[3 + c] - x[6 + c];
+  }
+  const R ru = norm3(u[0], u[1], u[2]), rv = norm3(v[0], v[1], v[2]);
+  const R iru = R(1) / ru, irv = R(1) / rv;
+  const R a1 = R(T.balpha[0]), a2 = R(T.balpha[1]), at = R(T.aalpha[0]);
+  const R d1 = ru - R(T.br0[0]), d2 = rv - R(T.br0[1]);
+  R uh[3], vh[3];
+#pragma unroll
+  for (int c = 0; c < 3; ++c) {
+    uh[c] = u[c] * iru;
+    vh[c] = v[c] * irv;
+  }
+  const R cth = clamp1(uh[0] * vh[0] + uh[1] * vh[1] + uh[2] * vh[2]);
+  const R sth = pvd_sqrt<R>(R(1) - cth * cth);
+  const R isth = R(1) / sth;
+  const R dq = pvd_acos<R>(cth) - R(T.ath0[0]);
+  lnpsi = R(T.c0) - R(0.5) * ((a1 * (d1 * d1) + a2 * (d2 * d2)) + at * (dq * dq));
+  // gradients: bonds -alpha dr r_hat; angle -alpha dq grad theta
+  const R gb1 = -a1 * d1, gb2 = -a2 * d2, ga = -at * dq;
+#pragma unroll
+  for (int c = 0; c < 3; ++c) {
+    const R tu = -(vh[c] - cth * uh[c]) * (iru * isth);  // d theta / d u
+    const R tv = -(uh[c] - cth * vh[c]) * (irv * isth);  // d theta / d v
+    g[c] = gb1 * uh[c] + ga * tu;
+    g[3 + c] = gb2 * vh[c] + ga * tv;
+    g[6 + c] = -(gb1 * uh[c] + gb2 * vh[c]) - ga * (tu + tv);
+  }
+  const R iru2 = iru * iru, irv2 = irv * irv, cot = cth * isth;
+  const R lb1 = -a1 * (R(1) + d1 * R(2) * iru), lb2 = -a2 * (R(1) + d2 * R(2) * irv);
+  const R la_i = -at * (iru2 + dq * cot * iru2);
+  const R la_k = -at * (irv2 + dq * cot * irv2);
+  const R la_j = -at * ((iru2 + irv2 - R(2) * cth * iru * irv) + dq * (cot * (iru2 + irv2) - R(2) * iru * irv * isth));
+  lap = im_h * (lb1 + lb2 + la_i + la_k) + im_o * (lb1 + lb2 + la_j);
+}
+
 template <typename R, int NP, int MS>
 PVD_DEV void trial_eval_s(const TrialParams& T, const R* x, TrialOut<R, NP>& o) {
   if constexpr ((MS & MS_TRIAL_WATER) != 0 && NP == 9) {
