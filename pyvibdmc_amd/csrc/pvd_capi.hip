@@ -10,6 +10,7 @@
 #include <cstdlib>
 #include <cstring>
 #include <string>
+#include <type_traits>
 #include <vector>
 
 #include "../../include/pyvibdmc_amd.h"
@@ -680,8 +681,20 @@ int do_branch_ext(pvd_sim* s, const double* v, const double* ub, hipStream_t st)
   A.inj_ubr = ub;
   unsigned g = nblk(s->cap, TILE);
   if (!s->cont) {
-    const void* fn = ub ? (s->is ? (const void*)k_discrete<R, 1, true, true, true> : (const void*)k_discrete<R, 1, true, false, true>)
-                        : (s->is ? (const void*)k_discrete<R, 1, false, true, true> : (const void*)k_discrete<R, 1, false, false, true>);
+    // compile-time component count where one is instantiated, else the run-time form (NP = 0)
+    const void* fn = nullptr;
+    auto pick = [&](auto np_tag) {
+      constexpr int NP = decltype(np_tag)::value;
+      fn = ub ? (s->is ? (const void*)k_discrete<R, NP, true, true, true> : (const void*)k_discrete<R, NP, true, false, true>)
+              : (s->is ? (const void*)k_discrete<R, NP, false, true, true> : (const void*)k_discrete<R, NP, false, false, true>);
+    };
+    switch (s->np) {
+      case 1: pick(std::integral_constant<int, 1>()); break;
+      case 3: pick(std::integral_constant<int, 3>()); break;
+      case 9: pick(std::integral_constant<int, 9>()); break;
+      case 18: pick(std::integral_constant<int, 18>()); break;
+      default: pick(std::integral_constant<int, 0>()); break;
+    }
     A.slots = disc_slots(s, fn);
     g = disc_grid(s, A.slots, disc_wpt<R, 1, true>());
     void* args[] = {&A};

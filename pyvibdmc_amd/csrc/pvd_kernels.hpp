@@ -573,7 +573,7 @@ PVD_DEV void finalize_discrete(const StepArgs<R>& A, Ctrl* C, int par, int n, in
 }
 
 // MODE: INJ = injected draws; IS = importance sampling (analytic trial); EXT = energies
-// from V_ext (walkers already moved), runtime component count.
+// from V_ext (walkers already moved; NP = 0 reads the component count at run time).
 // Tile b is block b.  Workgroups are dispatched in index order within each XCD, so the
 // predecessors a tile waits for are resident or finished; the wait is still bounded.
 template <typename R, int NP, bool INJ, bool IS, bool EXT, int MS = MS_ANY>
@@ -735,8 +735,19 @@ __global__ __launch_bounds__(TILE) void k_discrete(StepArgs<R> A) {
       for (int q = 0; q < c[k]; ++q) {
         const long long dst = base + q;
         if (dst >= A.cap) break;
-        if constexpr (EXT) {
+        if constexpr (EXT && NP == 0) {
           for (int p = 0; p < A.np; ++p) A.cout[p * A.cap + dst] = A.cin[p * A.cap + i];
+        } else if constexpr (EXT) {
+          // external energies: the walker is copied from the input planes
+          if constexpr (sizeof(R) == 4) {
+            const __amdgpu_buffer_rsrc_t ri = coord_rsrc((const float*)A.cin, A.cap, NP);
+            const __amdgpu_buffer_rsrc_t ro = coord_rsrc((const float*)A.cout, A.cap, NP);
+#pragma unroll
+            for (int p = 0; p < NP; ++p) coord_st(ro, A.cap, p, dst, coord_ld(ri, A.cap, p, i));
+          } else {
+#pragma unroll
+            for (int p = 0; p < NP; ++p) A.cout[p * A.cap + dst] = A.cin[p * A.cap + i];
+          }
         } else if constexpr (sizeof(R) == 4) {
           const __amdgpu_buffer_rsrc_t r = coord_rsrc((const float*)A.cout, A.cap, NP);
 #pragma unroll
