@@ -256,6 +256,33 @@ class GaussProductTrial:
             h += -a * (dq * dq + x * d2q)
         return g, g * g + h
 
+    def weighted_laplacian(self, cds, masses):
+        """sum_a (1/m_a) lap_a ln psi in closed form (the FP32 device kernels' formula):
+        bond r: |grad r|^2 = 1, lap r = 2/r at both atoms; angle theta with vertex j and arms
+        u, v: |grad_i|^2 = 1/ru^2, |grad_k|^2 = 1/rv^2, |grad_j|^2 = 1/ru^2 + 1/rv^2 -
+        2 cos/(ru rv); lap_i = cot/ru^2, lap_k = cot/rv^2, lap_j = cot (1/ru^2 + 1/rv^2) -
+        2/(ru rv sin).  Pinned against ``derivs`` (which central differences pin):
+        sum_p (1/m_p) d2psi/psi = sum_p (1/m_p) (d ln psi)^2 + weighted_laplacian."""
+        im = 1.0 / np.asarray(masses, np.float64)
+        lap = np.zeros(len(cds))
+        for (i, j, al, r0) in self.bonds:
+            u = cds[:, i] - cds[:, j]
+            r = np.linalg.norm(u, axis=1)
+            lap += -al * (1 + (r - r0) * 2 / r) * (im[i] + im[j])
+        for (i, j, k, al, t0) in self.angles:
+            u = cds[:, i] - cds[:, j]
+            v = cds[:, k] - cds[:, j]
+            ru = np.linalg.norm(u, axis=1)
+            rv = np.linalg.norm(v, axis=1)
+            c = np.clip((u * v).sum(1) / (ru * rv), -1, 1)
+            s = np.sqrt(1 - c * c)
+            cot = c / s
+            dq = np.arccos(c) - t0
+            iru2, irv2 = 1 / ru ** 2, 1 / rv ** 2
+            lap += (-al * (iru2 + dq * cot * iru2)) * im[i] + (-al * (irv2 + dq * cot * irv2)) * im[k]
+            lap += (-al * ((iru2 + irv2 - 2 * c / (ru * rv)) + dq * (cot * (iru2 + irv2) - 2 / (ru * rv * s)))) * im[j]
+        return lap
+
 
 def water_trial_params(m_h=M_H, m_o=M_O, De=MB_DE, a=MB_A, re=MB_RE, kb=MB_KB, theta0=MB_THETA0):
     """Harmonic Gaussian widths of the Morse+bend PES.

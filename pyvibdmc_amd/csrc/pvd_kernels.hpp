@@ -290,37 +290,52 @@ PVD_DEV R local_kin(const StepArgs<R>& A, const R* sec) {
 // Importance-sampled move of one walker (pyvibdmc.py:549-612, imp_samp.py:28-47).
 // On return x holds the new position and the function returns the local energy's
 // kinetic part at that position; `acc` tells whether the proposal was accepted.
+// FP32 importance-sampled move on a Gaussian-product trial: gradients and the closed-form
+// mass-weighted Laplacian only (trial_water_fast / trial_gauss_fast).  Same mathematics as the
+// literal path below (imp_samp.py:28-53): Metropolis in log form, local kinetic energy
+// -1/2 sum_p (1/m_p) (g_p^2 + d2 ln psi / dx_p^2) summed over components.
+template <typename R, int NP, int MS>
+PVD_DEV R is_move_fast(const StepArgs<R>& A, const R* disp, double u, R* x, bool& acc) {
+  const R dt = (R)A.dt;
+  R lx, ly, lapx, lapy, gx[NP], gy[NP], y[NP];
+  if constexpr ((MS & MS_TRIAL_WATER) != 0 && NP == 9) {
+    trial_water_fast<R>(A.trial, x, (R)A.inv_m[0], (R)A.inv_m[6], lx, gx, lapx);
+  } else {
+    trial_gauss_fast<R, NP>(A.trial, A.inv_m, x, lx, gx, lapx);
+  }
+#pragma unroll
+  for (int p = 0; p < NP; ++p) y[p] = (x[p] + disp[p]) + ((R)A.inv_m[p] * gx[p]) * dt;
+  if constexpr ((MS & MS_TRIAL_WATER) != 0 && NP == 9) {
+    trial_water_fast<R>(A.trial, y, (R)A.inv_m[0], (R)A.inv_m[6], ly, gy, lapy);
+  } else {
+    trial_gauss_fast<R, NP>(A.trial, A.inv_m, y, ly, gy, lapy);
+  }
+  R la = R(2) * (ly - lx);
+#pragma unroll
+  for (int p = 0; p < NP; ++p) {
+    const R a1 = (x[p] - y[p]) - ((R)A.inv_m[p] * gy[p]) * dt;
+    const R a2 = (y[p] - x[p]) - ((R)A.inv_m[p] * gx[p]) * dt;
+    la += (a2 * a2 - a1 * a1) * (R)A.h2s[p];
+  }
+  acc = (double)pvd_exp<R>(la) > u;
+  R gs = R(0);
+  if (acc) {
+#pragma unroll
+    for (int p = 0; p < NP; ++p) {
+      x[p] = y[p];
+      gs += (R)A.inv_m[p] * (gy[p] * gy[p]);
+    }
+    return R(-0.5) * (gs + lapy);
+  }
+#pragma unroll
+  for (int p = 0; p < NP; ++p) gs += (R)A.inv_m[p] * (gx[p] * gx[p]);
+  return R(-0.5) * (gs + lapx);
+}
+
 template <typename R, int NP, bool INJ, int MS = MS_ANY>
 PVD_DEV R is_move(const StepArgs<R>& A, const R* disp, double u, R* x, bool& acc) {
-  if constexpr ((MS & MS_TRIAL_WATER) != 0 && NP == 9 && sizeof(R) == 4) {
-    // FP32 water trial: gradients and the closed-form weighted Laplacian only
-    const R im_h = (R)A.inv_m[0], im_o = (R)A.inv_m[6];
-    const R dt = (R)A.dt;
-    R lx, ly, lapx, lapy, gx[9], gy[9], y[9];
-    trial_water_fast<R>(A.trial, x, im_h, im_o, lx, gx, lapx);
-#pragma unroll
-    for (int p = 0; p < 9; ++p) y[p] = (x[p] + disp[p]) + ((R)A.inv_m[p] * gx[p]) * dt;
-    trial_water_fast<R>(A.trial, y, im_h, im_o, ly, gy, lapy);
-    R la = R(2) * (ly - lx);
-#pragma unroll
-    for (int p = 0; p < 9; ++p) {
-      const R a1 = (x[p] - y[p]) - ((R)A.inv_m[p] * gy[p]) * dt;
-      const R a2 = (y[p] - x[p]) - ((R)A.inv_m[p] * gx[p]) * dt;
-      la += (a2 * a2 - a1 * a1) * (R)A.h2s[p];
-    }
-    acc = (double)pvd_exp<R>(la) > u;
-    R gs = R(0);
-    if (acc) {
-#pragma unroll
-      for (int p = 0; p < 9; ++p) {
-        x[p] = y[p];
-        gs += (R)A.inv_m[p] * (gy[p] * gy[p]);
-      }
-      return R(-0.5) * (gs + lapy);
-    }
-#pragma unroll
-    for (int p = 0; p < 9; ++p) gs += (R)A.inv_m[p] * (gx[p] * gx[p]);
-    return R(-0.5) * (gs + lapx);
+  if constexpr (sizeof(R) == 4 && NP % 3 == 0 && NP >= 6) {
+    if ((MS & MS_TRIAL_WATER) != 0 || A.trial.kind == TRIAL_GAUSS) return is_move_fast<R, NP, MS>(A, disp, u, x, acc);
   }
   TrialOut<R, NP> tx, ty;
   trial_eval_s<R, NP, MS>(A.trial, x, tx);

@@ -453,6 +453,96 @@ PVD_DEV R pes_eval_s(const PesParams& P, const R* x) {
   }
 }
 
+// Any Gaussian-product trial (runtime bond and angle lists, 3-D atoms) in the same closed
+// form as trial_water_fast: ln psi, g = grad ln psi and the mass-weighted Laplacian
+// lap = sum_a (1/m_a) lap_a ln psi (inv_m: 1/m per component, atom a at 3a).  Coordinates
+// are gathered and gradients scattered with selects over the NP components, so the walker
+// stays in registers.
+template <typename R, int NP>
+PVD_DEV void trial_gauss_fast(const TrialParams& T, const double* inv_m, const R* x, R& lnpsi, R* g, R& lap) {
+#pragma unroll
+  for (int p = 0; p < NP; ++p) g[p] = R(0);
+  R q2 = R(0);
+  lap = R(0);
+  auto gather = [&](int a, R* o) {
+#pragma unroll
+    for (int c = 0; c < 3; ++c) {
+      R v = R(0);
+#pragma unroll
+      for (int p = 0; p < NP; ++p)
+        if (p == 3 * a + c) v = x[p];
+      o[c] = v;
+    }
+  };
+  auto scatter = [&](int a, const R* d) {
+#pragma unroll
+    for (int c = 0; c < 3; ++c)
+#pragma unroll
+      for (int p = 0; p < NP; ++p)
+        if (p == 3 * a + c) g[p] += d[c];
+  };
+  for (int k = 0; k < T.nb; ++k) {
+    const int i = T.bi[k], j = T.bj[k];
+    R xi[3], xj[3], u[3];
+    gather(i, xi);
+    gather(j, xj);
+#pragma unroll
+    for (int c = 0; c < 3; ++c) u[c] = xi[c] - xj[c];
+    const R r = norm3(u[0], u[1], u[2]), ir = R(1) / r;
+    const R al = R(T.balpha[k]), dr = r - R(T.br0[k]);
+    q2 += al * (dr * dr);
+    R gi[3], gj[3];
+#pragma unroll
+    for (int c = 0; c < 3; ++c) {
+      gi[c] = -al * dr * (u[c] * ir);
+      gj[c] = -gi[c];
+    }
+    scatter(i, gi);
+    scatter(j, gj);
+    lap += -al * (R(1) + dr * R(2) * ir) * (R)(inv_m[3 * i] + inv_m[3 * j]);
+  }
+  for (int k = 0; k < T.na; ++k) {
+    const int i = T.ai[k], j = T.aj[k], kk = T.ak[k];
+    R xi[3], xj[3], xk[3], u[3], v[3];
+    gather(i, xi);
+    gather(j, xj);
+    gather(kk, xk);
+#pragma unroll
+    for (int c = 0; c < 3; ++c) {
+      u[c] = xi[c] - xj[c];
+      v[c] = xk[c] - xj[c];
+    }
+    const R ru = norm3(u[0], u[1], u[2]), rv = norm3(v[0], v[1], v[2]);
+    const R iru = R(1) / ru, irv = R(1) / rv;
+    R uh[3], vh[3];
+#pragma unroll
+    for (int c = 0; c < 3; ++c) {
+      uh[c] = u[c] * iru;
+      vh[c] = v[c] * irv;
+    }
+    const R cth = clamp1(uh[0] * vh[0] + uh[1] * vh[1] + uh[2] * vh[2]);
+    const R sth = pvd_sqrt<R>(R(1) - cth * cth), isth = R(1) / sth;
+    const R al = R(T.aalpha[k]), dq = pvd_acos<R>(cth) - R(T.ath0[k]);
+    q2 += al * (dq * dq);
+    R gi[3], gk[3], gj[3];
+#pragma unroll
+    for (int c = 0; c < 3; ++c) {
+      gi[c] = -al * dq * (-(vh[c] - cth * uh[c]) * (iru * isth));
+      gk[c] = -al * dq * (-(uh[c] - cth * vh[c]) * (irv * isth));
+      gj[c] = -(gi[c] + gk[c]);
+    }
+    scatter(i, gi);
+    scatter(kk, gk);
+    scatter(j, gj);
+    const R iru2 = iru * iru, irv2 = irv * irv, cot = cth * isth;
+    const R la_i = -al * (iru2 + dq * cot * iru2);
+    const R la_k = -al * (irv2 + dq * cot * irv2);
+    const R la_j = -al * ((iru2 + irv2 - R(2) * cth * iru * irv) + dq * (cot * (iru2 + irv2) - R(2) * iru * irv * isth));
+    lap += la_i * (R)inv_m[3 * i] + la_k * (R)inv_m[3 * kk] + la_j * (R)inv_m[3 * j];
+  }
+  lnpsi = R(T.c0) - R(0.5) * q2;
+}
+
 // The [H, H, O] Gaussian-product trial (bonds 0-2 and 1-2, angle 0-2-1) for importance
 // sampling: ln psi, g = grad ln psi (9 components) and the mass-weighted Laplacian
 // lap = sum_a (1/m_a) lap_a ln psi in closed form, so the local kinetic energy is

@@ -135,3 +135,21 @@ def test_oracle_c0_zpe_statistics():
     o.propagate()
     zpe = o.zpe_cm()
     assert abs(zpe - 1498.834) < 15.0
+
+
+@pytest.mark.parametrize('which', ['water', 'cluster'])
+def test_closed_form_weighted_laplacian_matches_per_component_derivatives(which):
+    """The closed-form sum_a (1/m_a) lap_a ln psi used by the FP32 importance-sampled move
+    equals the per-component sum of the derivatives the reference's trial functions return."""
+    if which == 'water':
+        tr, m = P.water_trial(), [P.M_H, P.M_H, P.M_O]
+        x = P.water_eq()[None] + np.random.RandomState(3).normal(0, 0.08, (500, 3, 3))
+    else:
+        tr, m = P.cluster6_trial(), [P.M_O, P.M_H, P.M_H, P.M_O, P.M_H, P.M_H]
+        x = P.cluster6_eq()[None] + np.random.RandomState(5).normal(0, 0.08, (500, 6, 3))
+    d1, d2 = tr.derivs(x)
+    im = np.repeat(1 / np.asarray(m), 3)
+    n = len(x)
+    ref = (im * d2.reshape(n, -1)).sum(1)
+    fast = (im * d1.reshape(n, -1) ** 2).sum(1) + tr.weighted_laplacian(x, m)
+    np.testing.assert_allclose(fast, ref, rtol=1e-11, atol=0)
