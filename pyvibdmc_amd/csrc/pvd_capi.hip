@@ -271,6 +271,9 @@ struct pvd_sim {
   int* klist;
   unsigned long long* killtag;
   double* rstage;
+  Cand* cand;
+  int* asg;
+  int* cpy;
   double* stage;
   void* parent;
   double* parent_w;
@@ -339,6 +342,9 @@ StepArgs<R> make_args(pvd_sim* s) {
   A.klist = s->klist;
   A.killtag = s->killtag;
   A.rstage = s->rstage;
+  A.cand = s->cand;
+  A.asg = s->asg;
+  A.cpy = s->cpy;
   std::memcpy(&A.pes, &s->cfg.pes, sizeof(PesParams));
   std::memcpy(&A.trial, &s->cfg.trial, sizeof(TrialParams));
   for (int p = 0; p < MAXNP; ++p) {
@@ -465,6 +471,7 @@ int step_kernels(pvd_sim* s, StepArgs<R>& A, Launch* L, int& nl) {
       PVD_ADD(k_cont<R, NP, INJ, false, false, (MS & MS_PES_WATER)>);
     }
     add((const void*)k_cont_replay<R>, 1, RP_NT);
+    add((const void*)k_cont_copy<R>, KILL_CAP / TILE, TILE);
   }
 #undef PVD_ADD
   return 0;
@@ -703,6 +710,7 @@ int do_branch_ext(pvd_sim* s, const double* v, const double* ub, hipStream_t st)
     if (s->is) hipLaunchKernelGGL((k_cont<R, 1, false, true, true>), dim3(g), dim3(TILE), 0, st, A);
     else hipLaunchKernelGGL((k_cont<R, 1, false, false, true>), dim3(g), dim3(TILE), 0, st, A);
     hipLaunchKernelGGL((k_cont_replay<R>), dim3(1), dim3(RP_NT), 0, st, A);
+    hipLaunchKernelGGL((k_cont_copy<R>), dim3(KILL_CAP / TILE), dim3(TILE), 0, st, A);
   }
   HIPCHK(hipGetLastError());
   s->next_step += 1;
@@ -776,7 +784,7 @@ void free_sim(pvd_sim* s) {
     if (s->graph[i]) (void)hipGraphExecDestroy(s->graph[i]);
   void* ptrs[] = {s->coords[0], s->coords[1], s->who[0], s->who[1], s->w, s->E, s->ke, s->kmask,
                   s->ctrl, s->flags, s->gflags, s->rec, s->drec, s->accs, s->tr_vref, s->tr_pop, s->tr_dteff, s->tr_births,
-                  s->tr_deaths, s->pend, s->fin, s->klist, s->killtag, s->rstage, s->stage, s->parent, s->parent_w, s->desc,
+                  s->tr_deaths, s->pend, s->fin, s->klist, s->killtag, s->rstage, s->cand, s->asg, s->cpy, s->stage, s->parent, s->parent_w, s->desc,
                   s->cnt, s->dacc, s->vsur, s->jlog};
   for (void* p : ptrs)
     if (p) (void)hipFree(p);
@@ -869,7 +877,10 @@ int pvd_sim_create(const pvd_config* cfg, pvd_sim** out) {
   if (s->cont) {
     ALLOC(s->pend, sizeof(Piece) * RP_PCAP * 2);
     ALLOC(s->fin, sizeof(Piece) * RP_FCAP);
-    ALLOC(s->klist, sizeof(int) * RP_KCAP);
+    ALLOC(s->klist, sizeof(int) * KILL_CAP);
+    ALLOC(s->cand, sizeof(Cand) * LIST_CAP);
+    ALLOC(s->asg, sizeof(int) * KILL_CAP);
+    ALLOC(s->cpy, sizeof(int) * 2 * KILL_CAP);
     ALLOC(s->killtag, sizeof(unsigned long long) * cap);
     ALLOC(s->rstage, sizeof(double) * (size_t)RP_FCAP * (np + 2));
   }

@@ -49,17 +49,34 @@ struct Ctrl {
   unsigned list_n;
   unsigned pad1;
   int ntd[2];
+  // continuous split replay: walkers with weight >= cand_t are appended to the candidate
+  // list by the weight kernel (0: no list)
+  double cand_t;
+  unsigned ncand;
+  int ncopy;  // split copies listed for the copy kernel (candidate replay), else 0
 };
 
 struct TileRec {
   double s0;  // discrete: sum c*E | continuous: sum w*E
   double s1;  // continuous: sum w | discrete: unused
+  double s2;  // continuous: sum w*E over the walkers not killed
+  double s3;  // continuous: sum w over the walkers not killed
   double mx;  // continuous: max weight in the tile
   int count;  // discrete: sum c   | continuous: kills
   int births;
   int deaths;
   int bad;
 };
+
+// A walker at or above the candidate threshold of the continuous split replay.
+struct Cand {
+  double w;
+  int slot;
+  int pad;
+};
+constexpr int CAND_CAP = 4096;   // candidates the replay sorts
+constexpr int LIST_CAP = 65536;  // candidate list entries per step
+constexpr int KILL_CAP = 32768;  // kills per step
 
 struct Piece {
   double v;
@@ -102,6 +119,9 @@ struct StepArgs {
   int* klist;              // kill list of the step (ascending slots)
   unsigned long long* killtag;  // per slot: (step << 32) | kill index, valid for this step only
   double* rstage;          // staged source walkers of the split copies
+  Cand* cand;              // candidate walkers of the split replay (unordered)
+  int* asg;                // per kill index: the candidate (sorted position) it copies
+  int* cpy;                // per kill index: (kill slot, source slot) for the copy kernel
   PesParams pes;
   TrialParams trial;
   double sigma[MAXNP];
@@ -928,14 +948,38 @@ __global__ __launch_bounds__(TILE) void k_cont(StepArgs<R> A) {
   }
   const unsigned long long km = __ballot(kill);
   if (lane_id() == 0) A.kmask[(long long)tile * (TILE / PVD_WAVE) + threadIdx.x / PVD_WAVE] = km;
+  // candidates of the split replay, appended in any order (the replay sorts them)
+  const double ct = C->cand_t;
+  if (A.branch && ct > 0.0) {
+    __shared__ int s_cb;
+    const bool cnd = valid && w >= ct;
+    int ctot;
+    const int cex = block_excl_scan<TILE>(cnd ? 1 : 0, s_i, ctot);
+    if (ctot > 0) {
+      if (threadIdx.x == 0) s_cb = (int)min(atomicAdd(&C->ncand, (unsigned)ctot), (unsigned)LIST_CAP);
+      __syncthreads();
+      const int o = s_cb + cex;
+      if (cnd && o < LIST_CAP) {
+        Cand c;
+        c.w = w;
+        c.slot = (int)i;
+        c.pad = 0;
+        A.cand[o] = c;
+      }
+    }
+  }
   const double swe = block_sum<TILE>(w * E, s_d);
   const double sw = block_sum<TILE>(w, s_d);
+  const double swe2 = block_sum<TILE>(kill ? 0.0 : w * E, s_d);
+  const double sw2 = block_sum<TILE>(kill ? 0.0 : w, s_d);
   const double mx = block_max<TILE>(valid ? w : 0.0, s_d);
   const int nk = block_sum<TILE>(kill ? 1 : 0, s_i);
   if (threadIdx.x == 0) {
     TileRec r;
     r.s0 = swe;
     r.s1 = sw;
+    r.s2 = swe2;
+    r.s3 = sw2;
     r.mx = mx;
     r.count = nk;
     r.births = 0;
@@ -964,7 +1008,7 @@ __global__ __launch_bounds__(TILE) void k_cont(StepArgs<R> A) {
 // sum(w*E) use the pre-split values.
 constexpr int RP_NT = 1024;
 constexpr int RP_CAP = 2048;   // interval members held in LDS
-constexpr int RP_KCAP = 4096;  // kills per step
+constexpr int RP_KCAP = KILL_CAP;  // kills per step
 constexpr int RP_PCAP = RP_KCAP * 2;  // pending pieces per buffer
 constexpr int RP_FCAP = RP_KCAP * 4;  // final pieces
 
@@ -989,13 +1033,420 @@ PVD_DEV bool piece_live(const Piece& e, int q, const unsigned long long* tag, un
   return !(kp < q && e.birth < kp);
 }
 
+// Outcome of the candidate replay; the interval replay runs when it is not RC_DONE.
+enum { RC_DONE = 0, RC_FEW = 1, RC_MANY = 2, RC_OTHER = 3 };
+// scratch: list keys and order (8 + 2 B), bucketed keys and order (8 + 2 B), bucket
+// counts (4 B per bucket, one per candidate), split depth (1 B)
+constexpr int RC_BUCKETS = CAND_CAP;
+constexpr int RC_LDS = 25 * CAND_CAP;
+constexpr int RP_LDS = RP_CAP * (int)sizeof(Piece) > RC_LDS ? RP_CAP * (int)sizeof(Piece) : RC_LDS;
+
+PVD_DEV int dexp(double w) { return (int)((__double_as_longlong(w) >> 52) & 0x7FF) - 1023; }
+
+// Histogram of weights for the candidate thresholds: 256 bins per octave from the lower
+// threshold up (the top 8 mantissa bits), 16 octaves, the last bin open-ended.
+constexpr int TH_SHIFT = 44;
+constexpr int TH_BINS = 4096;
+constexpr int TH_BPT = TH_BINS / RP_NT;  // bins per replay thread
+PVD_DEV int th_bin(double v, long long base) {
+  const long long b = (long long)((unsigned long long)__double_as_longlong(v) >> TH_SHIFT) - base;
+  return b < 0 ? -1 : (int)min(b, (long long)TH_BINS - 1);
+}
+PVD_DEV long long th_base(double thresh) {
+  return (long long)((unsigned long long)__double_as_longlong(thresh) >> TH_SHIFT);
+}
+PVD_DEV double th_edge(double thresh, int b) {
+  return fmax(thresh, __longlong_as_double((th_base(thresh) + b) << TH_SHIFT));
+}
+
+// Scan of the histogram from the top bin down: the first bin (counting down) at which the
+// running count passes `lim` (mode 0: reaches lim, mode 1: exceeds lim), or -1.  All
+// threads call it; thread t holds bins TH_BINS-1-TH_BPT*t .. TH_BINS-TH_BPT*(t+1).
+PVD_DEV int th_cross(const int* hist, int lim, bool exceed, int* s_i, int* s_sh) {
+  const int t = threadIdx.x;
+  int c[TH_BPT], loc = 0;
+#pragma unroll
+  for (int u = 0; u < TH_BPT; ++u) {
+    c[u] = hist[TH_BINS - 1 - (TH_BPT * t + u)];
+    loc += c[u];
+  }
+  if (t == 0) s_sh[2] = -1;
+  int tot;
+  int run = block_excl_scan<RP_NT>(loc, s_i, tot);
+#pragma unroll
+  for (int u = 0; u < TH_BPT; ++u) {
+    const bool hit = exceed ? (run <= lim && run + c[u] > lim) : (run < lim && run + c[u] >= lim);
+    if (hit) s_sh[2] = TH_BINS - 1 - (TH_BPT * t + u);
+    run += c[u];
+  }
+  __syncthreads();
+  const int b = s_sh[2];
+  __syncthreads();
+  return b;
+}
+
+// Lowest bin edge with at least `target` counts at or above it (a weight >= thresh), or 0
+// when the histogram holds fewer.
+PVD_DEV double th_pick(const int* hist, int target, double thresh, int* s_i, int* s_sh) {
+  const int b = th_cross(hist, target, false, s_i, s_sh);
+  return b < 0 ? 0.0 : th_edge(thresh, b);
+}
+
+// List entries the next step should hold: a few times the kills (weights near the cut
+// move by tens of per cent a step), well within the list.
+PVD_DEV int th_target(int K) { return min(max(2 * K, 2048), LIST_CAP / 4); }
+
+// Closed-form replay of the sequential split loop (pyvibdmc.py:344-356) from the
+// candidate list: the walkers of weight >= T0, where T0 >= the lower threshold.
+//
+// Each turn halves the largest entry (first slot on ties) and gives a copy with the other
+// half to the next killed slot.  An original walker o of weight w_o = m_o 2^e_o (m_o in
+// [1, 2)) that has been split d levels deep has therefore become 2^d entries of weight
+// w_o 2^-d, all copies of o.  The octave [2^E, 2^(E+1)) holds, for each o with e_o >= E,
+// its 2^(e_o-E) entries of the one value m_o 2^E: the loop takes the octaves from the top
+// down, the families of an octave in decreasing m_o, and the entries of one family in
+// slot order.  So a scan of the family sizes over the candidates sorted by mantissa gives,
+// octave by octave, the kill indices each family takes, until the K kills are used.  Every
+// kill slot then holds its family's walker with weight w_o 2^-D_o (D_o: the family's split
+// depth), and only the family p where the kills run out (r of its entries, in slot order)
+// needs its slots ranked.
+//
+// Splitting moves weight without changing any family's total, so the step's sums over
+// the walkers not killed are already those after the split (vref needs nothing from here).
+//
+// The replay uses the walkers at or above a cut T >= T0 (T0 itself when the list fits the
+// sort).  The entries of value >= T must number at least K: the chosen ones are then above
+// every other walker, and above every killed walker.  Equal mantissas of two families that
+// meet in an octave at or above the last one used, a list overflow or a split deeper than
+// the scratch holds make it return without touching the walkers.
+template <typename R>
+PVD_DEV int replay_candidates(const StepArgs<R>& A, int K, int nl, unsigned char* buf, int* s_i, double* s_d,
+                              int* s_sh, int* hist, double& t0n) {
+  constexpr int NT = RP_NT, JPT = CAND_CAP / RP_NT;
+  constexpr unsigned long long MANT = (1ull << 52) - 1;
+  const int tid = threadIdx.x;
+  if (nl <= 0) return RC_FEW;
+  if (nl > LIST_CAP) return RC_MANY;
+  if (K > KILL_CAP) return RC_OTHER;
+  const double T0 = A.ctrl->cand_t;
+  if (!(T0 >= A.thresh_lo) || !(T0 > 0.0)) return RC_OTHER;
+  const long long tb = th_base(A.thresh_lo);
+  unsigned long long* key = reinterpret_cast<unsigned long long*>(buf);
+  unsigned short* ord = reinterpret_cast<unsigned short*>(buf + 8 * CAND_CAP);
+  unsigned long long* bkey = reinterpret_cast<unsigned long long*>(buf + 10 * CAND_CAP);
+  unsigned short* bord = reinterpret_cast<unsigned short*>(buf + 18 * CAND_CAP);
+  int* bcnt = reinterpret_cast<int*>(buf + 20 * CAND_CAP);
+  unsigned char* dep = buf + 24 * CAND_CAP;
+  int* start = reinterpret_cast<int*>(buf);  // over the keys once they are sorted
+  // 1. the cut: T0 when the list fits the sort, else the lowest histogram edge with at most
+  //    CAND_CAP entries at or above it (the histogram is kept for the next threshold)
+  double T = T0;
+  const bool refined = nl > CAND_CAP;
+  if (refined) {
+    for (int b = tid; b < TH_BINS; b += NT) hist[b] = 0;
+    __syncthreads();
+    for (int q = tid; q < nl; q += NT) atomicAdd(&hist[th_bin(A.cand[q].w, tb)], 1);
+    __syncthreads();
+    const int over = th_cross(hist, CAND_CAP, true, s_i, s_sh);  // the bin that brings too many
+    if (over < 0 || over >= TH_BINS - 1) return RC_MANY;
+    T = fmax(T0, th_edge(A.thresh_lo, over + 1));
+  }
+  // the walkers >= T, in any order (wave ballots onto one counter)
+  if (tid == 0) s_sh[3] = 0;
+  __syncthreads();
+  for (int q0 = 0; q0 < nl; q0 += NT) {
+    const int q = q0 + tid;
+    double w = 0.0;
+    bool in = false;
+    if (q < nl) {
+      w = A.cand[q].w;
+      in = w >= T;
+    }
+    const unsigned long long bal = __ballot(in);
+    if (bal) {
+      const int lead = __ffsll((long long)bal) - 1;
+      int base = 0;
+      if (lane_id() == lead) base = atomicAdd(&s_sh[3], __popcll(bal));
+      base = __shfl(base, lead, PVD_WAVE);
+      const int o = base + __popcll(bal & ((1ull << lane_id()) - 1ull));
+      if (in && o < CAND_CAP) {
+        key[o] = MANT - ((unsigned long long)__double_as_longlong(w) & MANT);
+        ord[o] = (unsigned short)q;
+      }
+    }
+  }
+  __syncthreads();
+  const int C = s_sh[3];
+  if (C > CAND_CAP) return RC_MANY;
+  if (C == 0) return RC_FEW;
+  // 2. candidates by decreasing mantissa: counting sort on the top 12 bits of the key
+  //    (buckets in LDS), then each entry ranked inside its bucket; a bitonic sort when one
+  //    bucket holds too many
+  constexpr int KPT = CAND_CAP / NT;
+  constexpr int BSH = 52 - 12;
+  for (int b = tid; b < RC_BUCKETS; b += NT) bcnt[b] = 0;
+  __syncthreads();
+  int bpos[KPT];
+#pragma unroll
+  for (int k = 0; k < KPT; ++k) {
+    const int q = tid + k * NT;
+    bpos[k] = q < C ? atomicAdd(&bcnt[(int)(key[q] >> BSH)], 1) : 0;
+  }
+  __syncthreads();
+  int bmax;
+  {
+    int c4[4], loc = 0, mx = 0;
+#pragma unroll
+    for (int u = 0; u < 4; ++u) {
+      c4[u] = bcnt[tid * 4 + u];
+      loc += c4[u];
+      mx = max(mx, c4[u]);
+    }
+    int tot;
+    const int ex = block_excl_scan<NT>(loc, s_i, tot);
+    int st = ex;
+#pragma unroll
+    for (int u = 0; u < 4; ++u) {
+      bcnt[tid * 4 + u] = st;  // bucket starts (block_excl_scan's barriers: counts all read)
+      st += c4[u];
+    }
+    bmax = block_max<NT>(mx, s_i);
+  }
+  if (bmax <= 64) {
+#pragma unroll
+    for (int k = 0; k < KPT; ++k) {
+      const int q = tid + k * NT;
+      if (q < C) {
+        const int d = bcnt[(int)(key[q] >> BSH)] + bpos[k];
+        bkey[d] = key[q];
+        bord[d] = ord[q];
+      }
+    }
+    __syncthreads();
+#pragma unroll
+    for (int k = 0; k < KPT; ++k) {
+      const int q = tid + k * NT;
+      if (q < C) {
+        const unsigned long long kq = bkey[q];
+        const int bk = (int)(kq >> BSH);
+        const int bs = bcnt[bk], be = bk + 1 < RC_BUCKETS ? bcnt[bk + 1] : C;
+        int rank = 0;
+        for (int i = bs; i < be; ++i) {
+          const unsigned long long ki = bkey[i];
+          rank += (ki < kq || (ki == kq && i < q)) ? 1 : 0;
+        }
+        key[bs + rank] = kq;
+        ord[bs + rank] = bord[q];
+      }
+    }
+    __syncthreads();
+  } else {
+    int P = 64;
+    while (P < C) P <<= 1;
+    for (int q = C + tid; q < P; q += NT) {
+      key[q] = ~0ull;
+      ord[q] = 0;
+    }
+    __syncthreads();
+    for (int kk = 2; kk <= P; kk <<= 1) {
+      for (int jj = kk >> 1; jj > 0; jj >>= 1) {
+        for (int pi = tid; pi < P / 2; pi += NT) {
+          const int q = ((pi & ~(jj - 1)) << 1) | (pi & (jj - 1));
+          const int r = q | jj;
+          const unsigned long long a = key[q], b = key[r];
+          if (((q & kk) == 0) ? (a > b) : (a < b)) {
+            key[q] = b;
+            key[r] = a;
+            const unsigned short t = ord[q];
+            ord[q] = ord[r];
+            ord[r] = t;
+          }
+        }
+        __syncthreads();
+      }
+    }
+  }
+  // 3. this thread's families: sorted positions tid*JPT .. tid*JPT+JPT-1
+  double wj[JPT];
+  int oj[JPT], ej[JPT];
+  unsigned long long mj[JPT];
+  int etop = -0x40000000, tie_e = -0x40000000;
+#pragma unroll
+  for (int s = 0; s < JPT; ++s) {
+    const int j = tid * JPT + s;
+    wj[s] = 0.0;
+    oj[s] = -1;
+    ej[s] = -0x40000000;
+    mj[s] = 0;
+    if (j < C) {
+      const Cand c = A.cand[ord[j]];
+      wj[s] = c.w;
+      oj[s] = c.slot;
+      ej[s] = dexp(c.w);
+      mj[s] = (unsigned long long)__double_as_longlong(c.w) & MANT;
+      etop = max(etop, ej[s]);
+      if (j + 1 < C && key[j + 1] == key[j]) tie_e = max(tie_e, min(ej[s], dexp(A.cand[ord[j + 1]].w)));
+    }
+  }
+  etop = block_max<NT>(etop, s_i);
+  tie_e = block_max<NT>(tie_e, s_i);  // barriers: the keys are no longer read
+  const int eT = dexp(T);
+  const unsigned long long mT = (unsigned long long)__double_as_longlong(T) & MANT;
+  if (etop - eT > 40) return RC_OTHER;
+  // 4. octaves from the top: kill indices of each family, until K are used; in the octave
+  //    of T only the entries of value >= T count
+  int base = 0, Es = 0;
+  bool found = false;
+  for (int E = etop; E >= eT; --E) {
+    int sz[JPT], loc = 0;
+#pragma unroll
+    for (int s = 0; s < JPT; ++s) {
+      int v = 0;
+      if (ej[s] >= E && (E > eT || mj[s] >= mT)) {
+        const int d = ej[s] - E;
+        v = d >= 20 ? KILL_CAP + 1 : min(1 << d, KILL_CAP + 1);
+      }
+      sz[s] = v;
+      loc += v;
+    }
+    int tot;
+    const int ex = block_excl_scan<NT>(loc, s_i, tot);
+    int stj[JPT];
+    int st = base + ex;
+#pragma unroll
+    for (int s = 0; s < JPT; ++s) {
+      stj[s] = st;
+      if (tid * JPT + s < C) start[tid * JPT + s] = st;
+      st += sz[s];
+    }
+    __syncthreads();
+    const int hi = min(base + tot, K);
+    for (int t = base + tid; t < hi; t += NT) {
+      // the last family starting at or before t (families of size 0 start where the next does)
+      int lo = 0, h = C - 1;
+      while (lo < h) {
+        const int mid = (lo + h + 1) >> 1;
+        if (start[mid] <= t) lo = mid;
+        else h = mid - 1;
+      }
+      A.asg[t] = lo;
+    }
+    if (base + tot >= K) {
+#pragma unroll
+      for (int s = 0; s < JPT; ++s) {
+        if (sz[s] > 0 && stj[s] < K && stj[s] + sz[s] >= K) {
+          s_sh[0] = tid * JPT + s;
+          s_sh[1] = stj[s];
+        }
+      }
+      Es = E;
+      found = true;
+      __syncthreads();
+      break;
+    }
+    base += tot;
+  }
+  if (!found) return RC_FEW;
+  if (tie_e >= Es) return RC_OTHER;
+  const int p = s_sh[0], startp = s_sh[1];
+  const int r = K - startp;
+  const Cand cp = A.cand[ord[p]];
+  const int Dp = dexp(cp.w) - Es;
+  const int m = 1 << Dp;
+  if (Dp > 13) return RC_OTHER;  // the family's slots must fit the scratch
+  int Dj[JPT];
+#pragma unroll
+  for (int s = 0; s < JPT; ++s) {
+    const int j = tid * JPT + s;
+    Dj[s] = (j >= C || ej[s] < Es) ? 0 : ej[s] - Es + (j < p ? 1 : 0);
+    if (j < C) dep[j] = (unsigned char)Dj[s];
+  }
+  // 5. the slots of family p: the kill slots of its full levels, then its own
+  int* sp = start;
+  int cnt = 0;
+  for (int t0 = 0; t0 < startp; t0 += NT) {
+    const int t = t0 + tid;
+    const bool mem = t < startp && A.asg[t] == p;
+    int tot;
+    const int ex = block_excl_scan<NT>(mem ? 1 : 0, s_i, tot);
+    if (mem && cnt + ex < m) sp[cnt + ex] = A.klist[t];
+    cnt += tot;
+  }
+  if (cnt + 1 != m) return RC_OTHER;  // not reached: family p has 2^Dp entries
+  if (tid == 0) sp[m - 1] = cp.slot;
+  __syncthreads();
+  // 6. weights in place, and the list of copies (kill slot, source) for the copy kernel;
+  //    every source is a candidate, which is never a kill slot
+  for (int i = tid; i < m; i += NT) {
+    const int sl = sp[i];
+    int rank = 0;
+    for (int k = 0; k < m; ++k) rank += sp[k] < sl ? 1 : 0;
+    A.w[sl] = ldexp(cp.w, -(Dp + (rank < r ? 1 : 0)));
+  }
+  for (int t = tid; t < K; t += NT) {
+    const int j = A.asg[t];
+    const int ks = A.klist[t];
+    int o;
+    double wn;
+    if (j == p) {
+      o = cp.slot;
+      wn = ldexp(cp.w, -(Dp + 1));
+    } else {
+      const Cand c = A.cand[ord[j]];
+      o = c.slot;
+      wn = ldexp(c.w, -(int)dep[j]);
+    }
+    if (j != p || t >= startp) A.w[ks] = wn;
+    A.cpy[2 * t] = ks;
+    A.cpy[2 * t + 1] = o;
+  }
+#pragma unroll
+  for (int s = 0; s < JPT; ++s) {
+    const int j = tid * JPT + s;
+    if (j < C && j != p && Dj[s] > 0) A.w[oj[s]] = ldexp(wj[s], -Dj[s]);
+  }
+  // 7. next threshold: the walkers at or above T0 after the split are the list's walkers
+  //    below the cut (unchanged) and the families' entries (2^D_j copies of w_j 2^-D_j;
+  //    family p: m - r at w_p 2^-Dp, 2 r at half that)
+  //    (the list's histogram from step 1 with the split families moved, or empty)
+  if (!refined) {
+    for (int b = tid; b < TH_BINS; b += NT) hist[b] = 0;
+  }
+  __syncthreads();
+#pragma unroll
+  for (int s = 0; s < JPT; ++s) {
+    const int j = tid * JPT + s;
+    if (j >= C) continue;
+    if (refined) {
+      if (j != p && Dj[s] == 0) continue;  // counted at its unchanged weight
+      atomicSub(&hist[th_bin(wj[s], tb)], 1);
+    }
+    if (j != p) {
+      const double v = ldexp(wj[s], -Dj[s]);
+      if (v >= T0) atomicAdd(&hist[th_bin(v, tb)], 1 << Dj[s]);
+    } else {
+      const double v0 = ldexp(wj[s], -Dp), v1 = ldexp(wj[s], -(Dp + 1));
+      if (v0 >= T0 && m > r) atomicAdd(&hist[th_bin(v0, tb)], m - r);
+      if (v1 >= T0) atomicAdd(&hist[th_bin(v1, tb)], 2 * r);
+    }
+  }
+  __syncthreads();
+  t0n = th_pick(hist, th_target(K), A.thresh_lo, s_i, s_sh);
+  if (t0n == 0.0) t0n = fmax(A.thresh_lo, 0.75 * T0);  // fewer than the target above T0
+  return RC_DONE;
+}
+
 template <typename R>
 __global__ __launch_bounds__(RP_NT) void k_cont_replay(StepArgs<R> A) {
-  __shared__ Piece cur[RP_CAP];
+  __shared__ __align__(16) unsigned char s_buf[RP_LDS];
+  Piece* cur = reinterpret_cast<Piece*>(s_buf);
   __shared__ int s_cnt, s_ncur, s_npend, s_nfin, s_fail, s_next;
   __shared__ double s_d[RP_NT / PVD_WAVE];
   __shared__ int s_i[RP_NT / PVD_WAVE];
-  __shared__ int s_K;
+  __shared__ int s_K, s_nc;
+  __shared__ double s_a0n, s_a1n;
+  __shared__ int s_sh[4];
+  __shared__ int s_hist[TH_BINS];
   const int par = A.par;
   Ctrl* C = A.ctrl;
   const int np1 = par ^ 1;
@@ -1008,18 +1459,28 @@ __global__ __launch_bounds__(RP_NT) void k_cont_replay(StepArgs<R> A) {
   //    (calc_vref, continuous: pyvibdmc.py:659-661)
   {
     const unsigned acct = A.is_ ? acc_total(A, par) : 0u;
-    double a0 = 0.0, a1 = 0.0;
+    double a0 = 0.0, a1 = 0.0, a0n = 0.0, a1n = 0.0;
     int k = 0;
     for (int t = threadIdx.x; t < ntiles; t += RP_NT) {
       a0 += A.rec[t].s0;
       a1 += A.rec[t].s1;
+      a0n += A.rec[t].s2;
+      a1n += A.rec[t].s3;
       k += A.rec[t].count;
     }
     a0 = block_sum<RP_NT>(a0, s_d);
     a1 = block_sum<RP_NT>(a1, s_d);
+    a0n = block_sum<RP_NT>(a0n, s_d);
+    a1n = block_sum<RP_NT>(a1n, s_d);
     k = block_sum<RP_NT>(k, s_i);
     if (threadIdx.x == 0) {
       s_K = k;
+      s_a0n = a0n;
+      s_a1n = a1n;
+      C->ncopy = 0;
+      // the candidate count of this step; the next step's weight kernel starts from zero
+      s_nc = (int)min(C->ncand, (unsigned)LIST_CAP + 1u);
+      C->ncand = 0u;
       C->kills[par] = k;
       C->sumw[par] = a1;
       C->sumwe[par] = a0;
@@ -1043,6 +1504,7 @@ __global__ __launch_bounds__(RP_NT) void k_cont_replay(StepArgs<R> A) {
           A.tr_vref[step] = vnew;
           A.tr_pop[step] = a1;
           A.tr_births[step] = 0;
+          A.tr_deaths[step] = 0;
         }
       }
     }
@@ -1053,7 +1515,7 @@ __global__ __launch_bounds__(RP_NT) void k_cont_replay(StepArgs<R> A) {
   const unsigned long long tagbase = ep << 32;
   const int tid = threadIdx.x;
   int* kl = A.klist;
-  if (K > RP_KCAP) {
+  if (K > KILL_CAP) {
     if (tid == 0) {
       C->status = ST_BRANCH;
       C->status_step = (long long)step;
@@ -1076,16 +1538,50 @@ __global__ __launch_bounds__(RP_NT) void k_cont_replay(StepArgs<R> A) {
     while (mm) {
       const int b = __ffsll((long long)mm) - 1;
       mm &= mm - 1ull;
-      const int slot = wi * PVD_WAVE + b;
-      kl[o] = slot;
-      A.killtag[slot] = tagbase | (unsigned long long)o;
+      kl[o] = wi * PVD_WAVE + b;
       ++o;
     }
     __syncthreads();
     if (tid == 0) s_cnt += tot;
     __syncthreads();
   }
-  // 2. global maximum weight; first interval [gm, 2 gm)
+  // the step's vref from the sums after the split, and the next step's candidate threshold
+  auto finish = [&](double a0, double a1, double t0n, int rc) {
+    if (tid == 0) {
+      const double vnew = a0 / a1 - (A.alpha * ((a1 - A.n0) / A.n0));
+      C->vref[np1] = vnew;
+      if ((long long)step < A.tmax) {
+        A.tr_vref[step] = vnew;
+        A.tr_pop[step] = a1;
+        A.tr_births[step] = K;
+        A.tr_deaths[step] = 1 + rc;  // which replay ran (and why the list did not serve)
+      }
+      C->cand_t = t0n;
+    }
+  };
+  // 2. candidate replay; the interval replay below covers what it leaves
+  int rcode;
+  {
+    double t0n = 0.0;
+    rcode = replay_candidates(A, K, s_nc, s_buf, s_i, s_d, s_sh, s_hist, t0n);
+    if (rcode == RC_DONE) {
+      if (tid == 0) C->ncopy = K;
+      finish(s_a0n, s_a1n, t0n, rcode);
+      return;
+    }
+  }
+  __syncthreads();
+  // kill index of each killed slot, for the interval replay's validity rule
+  for (int q = tid; q < K; q += RP_NT) A.killtag[kl[q]] = tagbase | (unsigned long long)q;
+  __syncthreads();
+  if (K > RP_KCAP) {
+    if (tid == 0) {
+      C->status = ST_BRANCH;
+      C->status_step = (long long)step;
+    }
+    return;
+  }
+  // 3. global maximum weight; first interval [gm, 2 gm)
   double gm = 0.0;
   for (int t = tid; t < ntiles; t += RP_NT) gm = fmax(gm, A.rec[t].mx);
   gm = block_max<RP_NT>(gm, s_d);
@@ -1366,17 +1862,45 @@ __global__ __launch_bounds__(RP_NT) void k_cont_replay(StepArgs<R> A) {
       if (A.who_out) A.who_out[pc.slot] = (int)A.rstage[(long long)q * (np + 2) + np + 1];
     }
   }
-  if (tid == 0) {
-    const double a1 = C->sumw[par] + lsw;
-    const double a0 = C->sumwe[par] + lswe;
-    const double vnew = a0 / a1 - (A.alpha * ((a1 - A.n0) / A.n0));
-    C->vref[np1] = vnew;
-    if ((long long)step < A.tmax) {
-      A.tr_vref[step] = vnew;
-      A.tr_pop[step] = a1;
-      A.tr_births[step] = K;
+  // next threshold from every weight after the split (this path is the exception)
+  __syncthreads();
+  {
+    const long long tb = th_base(A.thresh_lo);
+    for (int b = tid; b < TH_BINS; b += RP_NT) s_hist[b] = 0;
+    __syncthreads();
+    for (int i = tid; i < n; i += RP_NT) {
+      const double w = A.w[i];
+      if (w >= A.thresh_lo) atomicAdd(&s_hist[th_bin(w, tb)], 1);
     }
+    __syncthreads();
   }
+  finish(C->sumwe[par] + lswe, C->sumw[par] + lsw, th_pick(s_hist, th_target(K), A.thresh_lo, s_i, s_sh), rcode);
+}
+
+// The split copies the candidate replay lists: kill slot <- source walker (coordinates,
+// energy, descendant index).  Sources are never kill slots, so the copies are independent
+// and spread over the whole chip.
+template <typename R>
+__global__ __launch_bounds__(TILE) void k_cont_copy(StepArgs<R> A) {
+  constexpr int B = 9;  // components per batch: all loads of a batch before its stores
+  const Ctrl* C = A.ctrl;
+  if (C->status != ST_OK) return;
+  const int t = blockIdx.x * TILE + threadIdx.x;
+  if (t >= C->ncopy) return;
+  const int ks = A.cpy[2 * t], o = A.cpy[2 * t + 1];
+  const double e = A.E[o];
+  const int who = A.who_out ? A.who_out[o] : 0;
+  for (int p0 = 0; p0 < A.np; p0 += B) {
+    R x[B];
+#pragma unroll
+    for (int q = 0; q < B; ++q)
+      if (p0 + q < A.np) x[q] = A.cout[(p0 + q) * A.cap + o];
+#pragma unroll
+    for (int q = 0; q < B; ++q)
+      if (p0 + q < A.np) A.cout[(p0 + q) * A.cap + ks] = x[q];
+  }
+  A.E[ks] = e;
+  if (A.who_out) A.who_out[ks] = who;
 }
 
 }  // namespace pvd

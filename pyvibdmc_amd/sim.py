@@ -328,7 +328,9 @@ class DMC_Sim:
             self.eff_ts[t0:t1] = e
 
     def trajectory_counts(self, t0, t1):
-        """births/deaths per step (discrete) or number of splits (continuous)."""
+        """births/deaths per step (discrete), or number of splits and the split replay that ran
+        (continuous: 0 none, 1 candidate list, 2-4 interval replay: list too short / too long /
+        other)."""
         cnt = t1 - t0
         b = np.empty(cnt, dtype=np.int32)
         dd = np.empty(cnt, dtype=np.int32)

@@ -91,3 +91,36 @@ def test_large_population_matches_oracle():
     np.testing.assert_array_equal(d, orc.rec['deaths'])
     coords, _ = sim._get_walkers()
     np.testing.assert_array_equal(coords, orc.coords)
+
+
+@pytest.mark.parametrize('n,steps', [(100_000, 6), (400_000, 2)])
+def test_continuous_large_population_matches_oracle(n, steps):
+    """Continuous weighting at n walkers, FP64, the reference's draws injected, start weights
+    log-uniform over [e^-4.8, e^2] so that about 3 % of the walkers fall below the 0.01
+    threshold in the first step (12 000 splits at 400k).  The first step has no candidate list
+    and runs the interval replay; the later ones take the list the weight kernel builds.
+    Splits per step, vref, weights and coordinates match the oracle's sequential
+    argmax/halve/clone loop (pyvibdmc.py:340-356)."""
+    import pyvibdmc_amd as pv
+    from oracle import pes as P
+    from oracle.dmc import LegacyDraws, OracleDMC
+    masses = [1837.15, 1837.15, 29156.95]
+    rs = np.random.RandomState(22)
+    start = P.water_eq()[None] + rs.normal(0, 0.05, (n, 3, 3))
+    w0 = np.exp(rs.uniform(-4.8, 2.0, n))
+    kw = dict(masses=masses, start_structures=start, num_walkers=n, num_timesteps=steps, delta_t=5,
+              equil_steps=10 ** 9, wfn_every=10 ** 9, weighting='continuous', cont_wt_thresh=0.01,
+              start_cont_wts=w0)
+    orc = OracleDMC(potential=P.morse_bend, draws=LegacyDraws(8), **kw)
+    orc.propagate()
+    sim = pv.DMC_Sim(atoms=['H', 'H', 'O'], potential=pv.MorseBendTriatomic(), precision='fp64',
+                     rng=LegacyDraws(8), write_files=False, chkpt_every=10 ** 9, **kw)
+    sim.propagate()
+    nb, how = sim.trajectory_counts(0, steps)
+    np.testing.assert_array_equal(nb, orc.rec['n_branched'])
+    assert nb[0] > 0.02 * n and how[0] == 2  # no list yet in the first step
+    assert (how[1:] == 1).all(), how
+    np.testing.assert_allclose(sim._vref_vs_tau, orc._vref_vs_tau, rtol=0, atol=1e-10)
+    coords, wts = sim._get_walkers()
+    np.testing.assert_array_equal(coords, orc.coords)
+    np.testing.assert_allclose(wts, orc.cont_wts, rtol=1e-12, atol=0)
