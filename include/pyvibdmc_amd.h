@@ -140,8 +140,9 @@ int pvd_sim_status(pvd_sim* sim, int32_t* status, int64_t* fail_step, int32_t* n
                    int64_t* next_step, void* stream);
 /* Per-step record (host arrays of `count`): vref_vs_tau, pop_vs_tau, births (discrete) or
  * number branched (continuous), deaths (discrete) or the split replay that ran (continuous:
- * 0 none, 1 from the candidate list, 2-4 the general interval replay because the list held
- * too few / too many walkers / other), cumulative effective
+ * 0 none; from the candidate lists: 1 list A whole, 5 list A with a cut inside, 6 lists A
+ * and B; 2-4 the general interval replay because the lists held too few / too many walkers /
+ * other), cumulative effective
  * time (IS) (pyvibdmc.py:674-682, 372-378). Any pointer may be NULL. */
 int pvd_sim_trajectory(pvd_sim* sim, int64_t t0, int32_t count, double* vref, double* pop,
                        int32_t* births, int32_t* deaths, double* eff_t);

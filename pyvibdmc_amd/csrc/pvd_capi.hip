@@ -878,7 +878,7 @@ int pvd_sim_create(const pvd_config* cfg, pvd_sim** out) {
     ALLOC(s->pend, sizeof(Piece) * RP_PCAP * 2);
     ALLOC(s->fin, sizeof(Piece) * RP_FCAP);
     ALLOC(s->klist, sizeof(int) * KILL_CAP);
-    ALLOC(s->cand, sizeof(Cand) * LIST_CAP);
+    ALLOC(s->cand, sizeof(Cand) * 2 * LIST_CAP);
     ALLOC(s->asg, sizeof(int) * KILL_CAP);
     ALLOC(s->cpy, sizeof(int) * 2 * KILL_CAP);
     ALLOC(s->killtag, sizeof(unsigned long long) * cap);

@@ -49,11 +49,20 @@ struct Ctrl {
   unsigned list_n;
   unsigned pad1;
   int ntd[2];
-  // continuous split replay: walkers with weight >= cand_t are appended to the candidate
-  // list by the weight kernel (0: no list)
+  // continuous split replay: the weight kernel appends the walkers of weight >= cand_t to
+  // list A and those in [cand_t2, cand_t) to list B (0: no lists; cand_t2 = cand_t: no
+  // list B), each in 8 sub-lists of
+  // SUB_CAP entries (by workgroup index) at cand + (list * 8 + sub) * SUB_CAP;
+  // cand_t = cut * cand_g,
+  // where cut is the last step's smallest halved weight and cand_g is fed back from the
+  // size of list A
   double cand_t;
-  unsigned ncand;
+  double cand_t2;
+  unsigned ncand[8];   // list A entries per sub-list (workgroup index mod 8)
+  unsigned ncand2[8];  // list B entries per sub-list
   int ncopy;  // split copies listed for the copy kernel (candidate replay), else 0
+  float cand_g;
+  int cand_b;  // steps list B stays on (it is kept after list A alone fell short)
 };
 
 struct TileRec {
@@ -75,7 +84,9 @@ struct Cand {
   int pad;
 };
 constexpr int CAND_CAP = 4096;   // candidates the replay sorts
-constexpr int LIST_CAP = 65536;  // candidate list entries per step
+constexpr int LIST_CAP = 32768;  // entries per candidate list and step
+constexpr int LIST_SUB = 8;      // sub-lists per list (spreads the appends' counters)
+constexpr int SUB_CAP = LIST_CAP / LIST_SUB;
 constexpr int KILL_CAP = 32768;  // kills per step
 
 struct Piece {
@@ -903,8 +914,6 @@ __global__ __launch_bounds__(TILE) void k_move(StepArgs<R> A) {
 // are flagged in per-wave masks for the split replay (pyvibdmc.py:436-437, 340-356).
 template <typename R, int NP, bool INJ, bool IS, bool EXT, int MS = MS_ANY>
 __global__ __launch_bounds__(TILE) void k_cont(StepArgs<R> A) {
-  __shared__ double s_d[TILE / PVD_WAVE];
-  __shared__ int s_i[TILE / PVD_WAVE];
   const int par = A.par;
   Ctrl* C = A.ctrl;
   const int n = C->n[par];
@@ -947,45 +956,72 @@ __global__ __launch_bounds__(TILE) void k_cont(StepArgs<R> A) {
     kill = A.branch && (w < A.thresh_lo);
   }
   const unsigned long long km = __ballot(kill);
-  if (lane_id() == 0) A.kmask[(long long)tile * (TILE / PVD_WAVE) + threadIdx.x / PVD_WAVE] = km;
-  // candidates of the split replay, appended in any order (the replay sorts them)
-  const double ct = C->cand_t;
-  if (A.branch && ct > 0.0) {
-    __shared__ int s_cb;
-    const bool cnd = valid && w >= ct;
-    int ctot;
-    const int cex = block_excl_scan<TILE>(cnd ? 1 : 0, s_i, ctot);
-    if (ctot > 0) {
-      if (threadIdx.x == 0) s_cb = (int)min(atomicAdd(&C->ncand, (unsigned)ctot), (unsigned)LIST_CAP);
-      __syncthreads();
-      const int o = s_cb + cex;
-      if (cnd && o < LIST_CAP) {
-        Cand c;
-        c.w = w;
-        c.slot = (int)i;
-        c.pad = 0;
-        A.cand[o] = c;
-      }
-    }
+  // candidates of the split replay, appended in any order (the replay sorts them): list A
+  // from cand_t up, list B from cand_t2 to cand_t
+  const double ct = C->cand_t, ct2 = C->cand_t2;
+  const bool cnd = A.branch && ct > 0.0 && valid && w >= ct;
+  const bool cnd2 = A.branch && ct2 > 0.0 && valid && !cnd && w >= ct2;
+  const unsigned long long cm = __ballot(cnd), cm2 = __ballot(cnd2);
+  // the tile's sums in one pass: wave partials, combined by one thread in a fixed order
+  constexpr int NW = TILE / PVD_WAVE;
+  __shared__ double s_r[5][NW];
+  __shared__ int s_k[NW], s_c[NW + 1], s_c2[NW + 1];
+  const int wv = threadIdx.x / PVD_WAVE, l = lane_id();
+  const double v0 = wave_sum(w * E), v1 = wave_sum(w);
+  const double v2 = wave_sum(kill ? 0.0 : w * E), v3 = wave_sum(kill ? 0.0 : w);
+  const double v4 = wave_max(valid ? w : 0.0);
+  if (l == 0) {
+    A.kmask[(long long)tile * NW + wv] = km;
+    s_r[0][wv] = v0;
+    s_r[1][wv] = v1;
+    s_r[2][wv] = v2;
+    s_r[3][wv] = v3;
+    s_r[4][wv] = v4;
+    s_k[wv] = __popcll(km);
+    s_c[wv] = __popcll(cm);
+    s_c2[wv] = __popcll(cm2);
   }
-  const double swe = block_sum<TILE>(w * E, s_d);
-  const double sw = block_sum<TILE>(w, s_d);
-  const double swe2 = block_sum<TILE>(kill ? 0.0 : w * E, s_d);
-  const double sw2 = block_sum<TILE>(kill ? 0.0 : w, s_d);
-  const double mx = block_max<TILE>(valid ? w : 0.0, s_d);
-  const int nk = block_sum<TILE>(kill ? 1 : 0, s_i);
+  __syncthreads();
   if (threadIdx.x == 0) {
     TileRec r;
-    r.s0 = swe;
-    r.s1 = sw;
-    r.s2 = swe2;
-    r.s3 = sw2;
-    r.mx = mx;
-    r.count = nk;
+    r.s0 = s_r[0][0];
+    r.s1 = s_r[1][0];
+    r.s2 = s_r[2][0];
+    r.s3 = s_r[3][0];
+    r.mx = s_r[4][0];
+    r.count = s_k[0];
+    int nc = s_c[0], nc2 = s_c2[0];
+    for (int q = 1; q < NW; ++q) {
+      r.s0 += s_r[0][q];
+      r.s1 += s_r[1][q];
+      r.s2 += s_r[2][q];
+      r.s3 += s_r[3][q];
+      r.mx = s_r[4][q] > r.mx ? s_r[4][q] : r.mx;
+      r.count += s_k[q];
+      nc += s_c[q];
+      nc2 += s_c2[q];
+    }
     r.births = 0;
     r.deaths = 0;
     r.bad = 0;
     A.rec[tile] = r;
+    const int sub = tile & (LIST_SUB - 1);
+    s_c[NW] = nc > 0 ? (int)min(atomicAdd(&C->ncand[sub], (unsigned)nc), (unsigned)SUB_CAP) : 0;
+    s_c2[NW] = nc2 > 0 ? (int)min(atomicAdd(&C->ncand2[sub], (unsigned)nc2), (unsigned)SUB_CAP) : 0;
+  }
+  __syncthreads();
+  if (cnd || cnd2) {
+    const unsigned long long bm = cnd ? cm : cm2;
+    const int* sc = cnd ? s_c : s_c2;
+    int o = sc[NW] + __popcll(bm & ((1ull << l) - 1ull));
+    for (int q = 0; q < wv; ++q) o += sc[q];
+    if (o < SUB_CAP) {
+      Cand c;
+      c.w = w;
+      c.slot = (int)i;
+      c.pad = 0;
+      A.cand[(cnd ? 0 : LIST_CAP) + (tile & (LIST_SUB - 1)) * SUB_CAP + o] = c;
+    }
   }
 }
 
@@ -1092,9 +1128,9 @@ PVD_DEV double th_pick(const int* hist, int target, double thresh, int* s_i, int
   return b < 0 ? 0.0 : th_edge(thresh, b);
 }
 
-// List entries the next step should hold: a few times the kills (weights near the cut
-// move by tens of per cent a step), well within the list.
-PVD_DEV int th_target(int K) { return min(max(2 * K, 2048), LIST_CAP / 4); }
+// List A size the threshold aims at: half as many again as the kills (at least 1024), so
+// that the replay needs no cut inside the list.
+PVD_DEV int th_target(int K) { return min(max(K + K / 2, 1024), CAND_CAP - CAND_CAP / 8); }
 
 // Closed-form replay of the sequential split loop (pyvibdmc.py:344-356) from the
 // candidate list: the walkers of weight >= T0, where T0 >= the lower threshold.
@@ -1120,16 +1156,34 @@ PVD_DEV int th_target(int K) { return min(max(2 * K, 2048), LIST_CAP / 4); }
 // meet in an octave at or above the last one used, a list overflow or a split deeper than
 // the scratch holds make it return without touching the walkers.
 template <typename R>
-PVD_DEV int replay_candidates(const StepArgs<R>& A, int K, int nl, unsigned char* buf, int* s_i, double* s_d,
-                              int* s_sh, int* hist, double& t0n) {
+PVD_DEV int replay_candidates(const StepArgs<R>& A, int K, int nlists, const int* subn, double T0, unsigned char* buf,
+                              int* s_i, int* s_sh, int* hist, double& cut, bool& refined) {
   constexpr int NT = RP_NT, JPT = CAND_CAP / RP_NT;
   constexpr unsigned long long MANT = (1ull << 52) - 1;
+  constexpr int NS = 2 * LIST_SUB;
   const int tid = threadIdx.x;
+  // the sub-lists used (list A's, then list B's), read as one list of nl entries
+  int pre[NS + 1];
+  pre[0] = 0;
+#pragma unroll
+  for (int c = 0; c < NS; ++c) {
+    const int k = c < nlists * LIST_SUB ? subn[c] : 0;
+    if (k > SUB_CAP) return RC_MANY;  // entries were lost
+    pre[c + 1] = pre[c] + k;
+  }
+  const int nl = pre[NS];
   if (nl <= 0) return RC_FEW;
-  if (nl > LIST_CAP) return RC_MANY;
   if (K > KILL_CAP) return RC_OTHER;
-  const double T0 = A.ctrl->cand_t;
   if (!(T0 >= A.thresh_lo) || !(T0 > 0.0)) return RC_OTHER;
+  // position in the candidate buffer of the q-th entry (constant indices only: the
+  // sub-list bounds stay in registers; an empty sub-list is passed over)
+  auto phys = [&](int q) -> int {
+    int off = 0;
+#pragma unroll
+    for (int u = 0; u < NS; ++u)
+      if (q >= pre[u]) off = u * SUB_CAP - pre[u];
+    return q + off;
+  };
   const long long tb = th_base(A.thresh_lo);
   unsigned long long* key = reinterpret_cast<unsigned long long*>(buf);
   unsigned short* ord = reinterpret_cast<unsigned short*>(buf + 8 * CAND_CAP);
@@ -1139,15 +1193,17 @@ PVD_DEV int replay_candidates(const StepArgs<R>& A, int K, int nl, unsigned char
   unsigned char* dep = buf + 24 * CAND_CAP;
   int* start = reinterpret_cast<int*>(buf);  // over the keys once they are sorted
   // 1. the cut: T0 when the list fits the sort, else the lowest histogram edge with at most
-  //    CAND_CAP entries at or above it (the histogram is kept for the next threshold)
+  //    about 1.25 K entries at or above it
   double T = T0;
-  const bool refined = nl > CAND_CAP;
+  refined = nl > CAND_CAP;
   if (refined) {
     for (int b = tid; b < TH_BINS; b += NT) hist[b] = 0;
     __syncthreads();
-    for (int q = tid; q < nl; q += NT) atomicAdd(&hist[th_bin(A.cand[q].w, tb)], 1);
+    for (int q = tid; q < nl; q += NT) atomicAdd(&hist[th_bin(A.cand[phys(q)].w, tb)], 1);
     __syncthreads();
-    const int over = th_cross(hist, CAND_CAP, true, s_i, s_sh);  // the bin that brings too many
+    // about 1.25 K walkers (at least 1024): they alone hold K entries
+    const int lim = min(CAND_CAP, max(K + K / 4, 1024));
+    const int over = th_cross(hist, lim, true, s_i, s_sh);  // the bin that brings too many
     if (over < 0 || over >= TH_BINS - 1) return RC_MANY;
     T = fmax(T0, th_edge(A.thresh_lo, over + 1));
   }
@@ -1156,10 +1212,11 @@ PVD_DEV int replay_candidates(const StepArgs<R>& A, int K, int nl, unsigned char
   __syncthreads();
   for (int q0 = 0; q0 < nl; q0 += NT) {
     const int q = q0 + tid;
+    const int pq = phys(q);
     double w = 0.0;
     bool in = false;
     if (q < nl) {
-      w = A.cand[q].w;
+      w = A.cand[pq].w;
       in = w >= T;
     }
     const unsigned long long bal = __ballot(in);
@@ -1171,7 +1228,7 @@ PVD_DEV int replay_candidates(const StepArgs<R>& A, int K, int nl, unsigned char
       const int o = base + __popcll(bal & ((1ull << lane_id()) - 1ull));
       if (in && o < CAND_CAP) {
         key[o] = MANT - ((unsigned long long)__double_as_longlong(w) & MANT);
-        ord[o] = (unsigned short)q;
+        ord[o] = (unsigned short)pq;  // < 2 * LIST_CAP = 65536
       }
     }
   }
@@ -1405,34 +1462,7 @@ PVD_DEV int replay_candidates(const StepArgs<R>& A, int K, int nl, unsigned char
     const int j = tid * JPT + s;
     if (j < C && j != p && Dj[s] > 0) A.w[oj[s]] = ldexp(wj[s], -Dj[s]);
   }
-  // 7. next threshold: the walkers at or above T0 after the split are the list's walkers
-  //    below the cut (unchanged) and the families' entries (2^D_j copies of w_j 2^-D_j;
-  //    family p: m - r at w_p 2^-Dp, 2 r at half that)
-  //    (the list's histogram from step 1 with the split families moved, or empty)
-  if (!refined) {
-    for (int b = tid; b < TH_BINS; b += NT) hist[b] = 0;
-  }
-  __syncthreads();
-#pragma unroll
-  for (int s = 0; s < JPT; ++s) {
-    const int j = tid * JPT + s;
-    if (j >= C) continue;
-    if (refined) {
-      if (j != p && Dj[s] == 0) continue;  // counted at its unchanged weight
-      atomicSub(&hist[th_bin(wj[s], tb)], 1);
-    }
-    if (j != p) {
-      const double v = ldexp(wj[s], -Dj[s]);
-      if (v >= T0) atomicAdd(&hist[th_bin(v, tb)], 1 << Dj[s]);
-    } else {
-      const double v0 = ldexp(wj[s], -Dp), v1 = ldexp(wj[s], -(Dp + 1));
-      if (v0 >= T0 && m > r) atomicAdd(&hist[th_bin(v0, tb)], m - r);
-      if (v1 >= T0) atomicAdd(&hist[th_bin(v1, tb)], 2 * r);
-    }
-  }
-  __syncthreads();
-  t0n = th_pick(hist, th_target(K), A.thresh_lo, s_i, s_sh);
-  if (t0n == 0.0) t0n = fmax(A.thresh_lo, 0.75 * T0);  // fewer than the target above T0
+  cut = ldexp(cp.w, -Dp);
   return RC_DONE;
 }
 
@@ -1440,10 +1470,11 @@ template <typename R>
 __global__ __launch_bounds__(RP_NT) void k_cont_replay(StepArgs<R> A) {
   __shared__ __align__(16) unsigned char s_buf[RP_LDS];
   Piece* cur = reinterpret_cast<Piece*>(s_buf);
-  __shared__ int s_cnt, s_ncur, s_npend, s_nfin, s_fail, s_next;
+  __shared__ int s_ncur, s_npend, s_nfin, s_fail, s_next;
   __shared__ double s_d[RP_NT / PVD_WAVE];
   __shared__ int s_i[RP_NT / PVD_WAVE];
   __shared__ int s_K, s_nc;
+  __shared__ int s_subn[2 * LIST_SUB];
   __shared__ double s_a0n, s_a1n;
   __shared__ int s_sh[4];
   __shared__ int s_hist[TH_BINS];
@@ -1455,32 +1486,91 @@ __global__ __launch_bounds__(RP_NT) void k_cont_replay(StepArgs<R> A) {
   const int ntiles = (n + TILE - 1) / TILE;
   const unsigned long long step = C->step[par];
   const unsigned long long ep = (unsigned long long)C->epoch[par];
-  // 0. fixed-order reduction of the per-tile sums; without splits this finishes the step
-  //    (calc_vref, continuous: pyvibdmc.py:659-661)
+  const int tid = threadIdx.x;
+  int* kl = A.klist;
+  // 0. the kill list in ascending slot order (from the per-wave masks) and the fixed-order
+  //    sums of the tile records, their loads issued together; without splits the sums
+  //    finish the step (calc_vref, continuous: pyvibdmc.py:659-661)
   {
     const unsigned acct = A.is_ ? acc_total(A, par) : 0u;
+    const int nwords = ntiles * (TILE / PVD_WAVE);
+    constexpr int KW = 16;  // mask words per thread and round
+    int kbase = 0;
     double a0 = 0.0, a1 = 0.0, a0n = 0.0, a1n = 0.0;
-    int k = 0;
-    for (int t = threadIdx.x; t < ntiles; t += RP_NT) {
-      a0 += A.rec[t].s0;
-      a1 += A.rec[t].s1;
-      a0n += A.rec[t].s2;
-      a1n += A.rec[t].s3;
-      k += A.rec[t].count;
+    for (int base = 0; base < nwords || base == 0; base += RP_NT * KW) {
+      const int span = min(nwords - base, RP_NT * KW);
+      const int per = (span + RP_NT - 1) / RP_NT;
+      const int w0 = base + tid * per;
+      unsigned long long m[KW];
+      int pc = 0;
+#pragma unroll
+      for (int u = 0; u < KW; ++u) {
+        m[u] = (u < per && w0 + u < base + span) ? A.kmask[w0 + u] : 0ull;
+        pc += __popcll(m[u]);
+      }
+      if (base == 0) {
+        for (int t = tid; t < ntiles; t += RP_NT) {
+          const TileRec& rc = A.rec[t];
+          a0 += rc.s0;
+          a1 += rc.s1;
+          a0n += rc.s2;
+          a1n += rc.s3;
+        }
+      }
+      int tot;
+      const int ex = block_excl_scan<RP_NT>(pc, s_i, tot);
+      int o = kbase + ex;
+#pragma unroll
+      for (int u = 0; u < KW; ++u) {
+        unsigned long long mm = m[u];
+        while (mm) {
+          const int b = __ffsll((long long)mm) - 1;
+          mm &= mm - 1ull;
+          if (o < KILL_CAP) kl[o] = (w0 + u) * PVD_WAVE + b;
+          ++o;
+        }
+      }
+      kbase += tot;
     }
-    a0 = block_sum<RP_NT>(a0, s_d);
-    a1 = block_sum<RP_NT>(a1, s_d);
-    a0n = block_sum<RP_NT>(a0n, s_d);
-    a1n = block_sum<RP_NT>(a1n, s_d);
-    k = block_sum<RP_NT>(k, s_i);
-    if (threadIdx.x == 0) {
+    const int k = kbase;
+    {
+      // one pass: wave partials, combined by one thread in a fixed order
+      constexpr int NW = RP_NT / PVD_WAVE;
+      const int wv = tid / PVD_WAVE;
+      const double v0 = wave_sum(a0), v1 = wave_sum(a1), v2 = wave_sum(a0n), v3 = wave_sum(a1n);
+      double* sr = reinterpret_cast<double*>(s_buf);
+      if (lane_id() == 0) {
+        sr[wv] = v0;
+        sr[NW + wv] = v1;
+        sr[2 * NW + wv] = v2;
+        sr[3 * NW + wv] = v3;
+      }
+      __syncthreads();
+      a0 = sr[0];
+      a1 = sr[NW];
+      a0n = sr[2 * NW];
+      a1n = sr[3 * NW];
+      for (int q = 1; q < NW; ++q) {
+        a0 += sr[q];
+        a1 += sr[NW + q];
+        a0n += sr[2 * NW + q];
+        a1n += sr[3 * NW + q];
+      }
+    }
+    if (tid == 0) {
       s_K = k;
       s_a0n = a0n;
       s_a1n = a1n;
       C->ncopy = 0;
       // the candidate count of this step; the next step's weight kernel starts from zero
-      s_nc = (int)min(C->ncand, (unsigned)LIST_CAP + 1u);
-      C->ncand = 0u;
+      s_nc = 0;
+      for (int c = 0; c < LIST_SUB; ++c) {
+        s_subn[c] = (int)min(C->ncand[c], (unsigned)SUB_CAP + 1u);
+        s_subn[LIST_SUB + c] = (int)min(C->ncand2[c], (unsigned)SUB_CAP + 1u);
+        s_nc += s_subn[c];
+        C->ncand[c] = 0u;
+        C->ncand2[c] = 0u;
+      }
       C->kills[par] = k;
       C->sumw[par] = a1;
       C->sumwe[par] = a0;
@@ -1492,8 +1582,8 @@ __global__ __launch_bounds__(RP_NT) void k_cont_replay(StepArgs<R> A) {
         C->eff_t = (step == 0 ? 0.0 : C->eff_t) + dte;
         A.tr_dteff[step] = C->eff_t;
       }
-      // the step's weights and every thread here have read the acceptance count (the block
-      // sums above hold barriers): clear it for step + 2
+      // the step's weights and every thread here have read the acceptance count (the scans
+      // above hold barriers): clear it for step + 2
       if (A.is_) {
         for (int b = 0; b < ACC_SLOTS; ++b) st_relaxed_u(acc_slot(A, par, b), 0u);
       }
@@ -1513,8 +1603,6 @@ __global__ __launch_bounds__(RP_NT) void k_cont_replay(StepArgs<R> A) {
   const int K = s_K;
   if (K == 0) return;
   const unsigned long long tagbase = ep << 32;
-  const int tid = threadIdx.x;
-  int* kl = A.klist;
   if (K > KILL_CAP) {
     if (tid == 0) {
       C->status = ST_BRANCH;
@@ -1522,31 +1610,10 @@ __global__ __launch_bounds__(RP_NT) void k_cont_replay(StepArgs<R> A) {
     }
     return;
   }
-  // 1. kill list in ascending slot order from the per-wave masks, and each killed
-  //    slot's position in it
-  const int nwords = ntiles * (TILE / PVD_WAVE);
-  if (tid == 0) s_cnt = 0;
-  __syncthreads();
-  for (int base = 0; base < nwords; base += RP_NT) {
-    const int wi = base + tid;
-    const unsigned long long m = wi < nwords ? A.kmask[wi] : 0ull;
-    const int pc = __popcll(m);
-    int tot;
-    const int ex = block_excl_scan<RP_NT>(pc, s_i, tot);
-    unsigned long long mm = m;
-    int o = s_cnt + ex;
-    while (mm) {
-      const int b = __ffsll((long long)mm) - 1;
-      mm &= mm - 1ull;
-      kl[o] = wi * PVD_WAVE + b;
-      ++o;
-    }
-    __syncthreads();
-    if (tid == 0) s_cnt += tot;
-    __syncthreads();
-  }
-  // the step's vref from the sums after the split, and the next step's candidate threshold
-  auto finish = [&](double a0, double a1, double t0n, int rc) {
+  // the step's vref from the sums after the split, and the next step's lists: the
+  // threshold follows this step's cut, its factor moved by (size of list A / wanted)^0.2,
+  // or, after the interval replay, is read off the histogram of all weights (t0 > 0)
+  auto finish = [&](double a0, double a1, double cut, int code, double t0, bool miss) {
     if (tid == 0) {
       const double vnew = a0 / a1 - (A.alpha * ((a1 - A.n0) / A.n0));
       C->vref[np1] = vnew;
@@ -1554,19 +1621,35 @@ __global__ __launch_bounds__(RP_NT) void k_cont_replay(StepArgs<R> A) {
         A.tr_vref[step] = vnew;
         A.tr_pop[step] = a1;
         A.tr_births[step] = K;
-        A.tr_deaths[step] = 1 + rc;  // which replay ran (and why the list did not serve)
+        A.tr_deaths[step] = code;  // which replay ran (and why the lists did not serve)
       }
-      C->cand_t = t0n;
+      float g = C->cand_g;
+      if (!(g > 0.0f)) g = 0.5f;
+      if (t0 > 0.0 && cut > 0.0) g = (float)(t0 / cut);
+      else if (C->cand_t > 0.0) g *= s_nc > 0 ? powf((float)s_nc / (float)th_target(K), 0.2f) : 0.5f;
+      g = fminf(0.98f, fmaxf(0.02f, g));
+      C->cand_g = g;
+      C->cand_t = cut > 0.0 ? fmax(A.thresh_lo, cut * g) : 0.0;
+      const int nb = miss ? 64 : max(0, C->cand_b - 1);
+      C->cand_b = nb;
+      C->cand_t2 = (cut > 0.0 && nb > 0) ? fmax(A.thresh_lo, 0.75 * cut * g) : C->cand_t;
     }
   };
-  // 2. candidate replay; the interval replay below covers what it leaves
+  // 2. candidate replay from list A, else from lists A and B; the interval replay below
+  //    covers what they leave
   int rcode;
   {
-    double t0n = 0.0;
-    rcode = replay_candidates(A, K, s_nc, s_buf, s_i, s_d, s_sh, s_hist, t0n);
+    double cut = 0.0;
+    bool refined = false;
+    rcode = replay_candidates(A, K, 1, s_subn, C->cand_t, s_buf, s_i, s_sh, s_hist, cut, refined);
+    const bool miss = rcode != RC_DONE;
+    if (rcode == RC_FEW && C->cand_t2 < C->cand_t) {
+      __syncthreads();
+      rcode = replay_candidates(A, K, 2, s_subn, C->cand_t2, s_buf, s_i, s_sh, s_hist, cut, refined);
+    }
     if (rcode == RC_DONE) {
       if (tid == 0) C->ncopy = K;
-      finish(s_a0n, s_a1n, t0n, rcode);
+      finish(s_a0n, s_a1n, cut, miss ? 6 : refined ? 5 : 1, 0.0, miss);
       return;
     }
   }
@@ -1862,7 +1945,7 @@ __global__ __launch_bounds__(RP_NT) void k_cont_replay(StepArgs<R> A) {
       if (A.who_out) A.who_out[pc.slot] = (int)A.rstage[(long long)q * (np + 2) + np + 1];
     }
   }
-  // next threshold from every weight after the split (this path is the exception)
+  // the next threshold from every weight after the split (this path is the exception)
   __syncthreads();
   {
     const long long tb = th_base(A.thresh_lo);
@@ -1874,7 +1957,8 @@ __global__ __launch_bounds__(RP_NT) void k_cont_replay(StepArgs<R> A) {
     }
     __syncthreads();
   }
-  finish(C->sumwe[par] + lswe, C->sumw[par] + lsw, th_pick(s_hist, th_target(K), A.thresh_lo, s_i, s_sh), rcode);
+  finish(C->sumwe[par] + lswe, C->sumw[par] + lsw, H, 1 + rcode,
+         th_pick(s_hist, th_target(K), A.thresh_lo, s_i, s_sh), true);
 }
 
 // The split copies the candidate replay lists: kill slot <- source walker (coordinates,
@@ -1884,10 +1968,9 @@ template <typename R>
 __global__ __launch_bounds__(TILE) void k_cont_copy(StepArgs<R> A) {
   constexpr int B = 9;  // components per batch: all loads of a batch before its stores
   const Ctrl* C = A.ctrl;
-  if (C->status != ST_OK) return;
-  const int t = blockIdx.x * TILE + threadIdx.x;
-  if (t >= C->ncopy) return;
+  const int t = blockIdx.x * TILE + threadIdx.x;  // < KILL_CAP: the list load is in bounds
   const int ks = A.cpy[2 * t], o = A.cpy[2 * t + 1];
+  if (C->status != ST_OK || t >= C->ncopy) return;
   const double e = A.E[o];
   const int who = A.who_out ? A.who_out[o] : 0;
   for (int p0 = 0; p0 < A.np; p0 += B) {

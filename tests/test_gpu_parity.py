@@ -93,14 +93,14 @@ def test_large_population_matches_oracle():
     np.testing.assert_array_equal(coords, orc.coords)
 
 
-@pytest.mark.parametrize('n,steps', [(100_000, 6), (400_000, 2)])
+@pytest.mark.parametrize('n,steps', [(100_000, 120), (400_000, 2)])
 def test_continuous_large_population_matches_oracle(n, steps):
     """Continuous weighting at n walkers, FP64, the reference's draws injected, start weights
     log-uniform over [e^-4.8, e^2] so that about 3 % of the walkers fall below the 0.01
-    threshold in the first step (12 000 splits at 400k).  The first step has no candidate list
-    and runs the interval replay; the later ones take the list the weight kernel builds.
-    Splits per step, vref, weights and coordinates match the oracle's sequential
-    argmax/halve/clone loop (pyvibdmc.py:340-356)."""
+    threshold in the first step (12 000 splits at 400k), a few hundred a step after that.
+    The first step has no candidate list and runs the interval replay; the later ones take
+    the lists the weight kernel builds.  Splits per step, vref, weights and coordinates match
+    the oracle's sequential argmax/halve/clone loop (pyvibdmc.py:340-356)."""
     import pyvibdmc_amd as pv
     from oracle import pes as P
     from oracle.dmc import LegacyDraws, OracleDMC
@@ -119,7 +119,7 @@ def test_continuous_large_population_matches_oracle(n, steps):
     nb, how = sim.trajectory_counts(0, steps)
     np.testing.assert_array_equal(nb, orc.rec['n_branched'])
     assert nb[0] > 0.02 * n and how[0] == 2  # no list yet in the first step
-    assert (how[1:] == 1).all(), how
+    assert np.isin(how[1:], [1, 5, 6]).all(), how
     np.testing.assert_allclose(sim._vref_vs_tau, orc._vref_vs_tau, rtol=0, atol=1e-10)
     coords, wts = sim._get_walkers()
     np.testing.assert_array_equal(coords, orc.coords)
