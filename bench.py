@@ -9,7 +9,9 @@ branching, compaction, vref in the last tile); value = walker-steps/s = sum over
 population entering each step, divided by the wall time between two device syncs.
 
 Also reported: the same case with importance sampling (Gaussian-product psi_T, drift and
-local energy from the trial kernels, Metropolis); the surrogate form (an MLP trained on the
+local energy from the trial kernels, Metropolis); the continuous-weighting workload (100k
+walkers, 10 000 steps, exact split replay, descendant weighting) in FP32 and FP64; the
+surrogate form (an MLP trained on the
 GPU on 2M geometries an analytic importance-sampled run visits, then driving the same
 importance-sampled DMC); ZPE runs of every form against FP64 runs; the dominant kernel's
 roofline fraction (algorithmic bytes / average launch duration from HIP events, against
@@ -165,6 +167,53 @@ def run_timed(sim, warmup, steps, events_steps=0):
     return ws, wall, float(np.mean(entering)), kern_ms, ev_n
 
 
+def continuous_variant(n=100_000, T=10_000):
+    """The continuous-weighting workload: the triatomic PES, n walkers started at the
+    equilibrium geometry + N(0, 0.05 bohr) (seed 0), dt = 5, T steps, lower weight threshold
+    0.01 (walkers below it are replaced by splitting the heaviest, pyvibdmc.py:340-356),
+    descendant weighting every 500 steps over 50 steps, in FP32 and FP64.  Reports
+    walker-steps/s over the whole propagate (host events of the wave-function saves
+    included), each precision's ZPE, which split replay each step took, and the step's three
+    kernels from HIP events over 200 further steps, with the weight kernel's HBM roofline."""
+    import ctypes
+    import torch
+    import pyvibdmc_amd as pv
+    from pyvibdmc_amd import _native as N
+    t = np.deg2rad(104.5)
+    eq = np.array([[1.81, 0.0, 0.0], [1.81 * np.cos(t), 1.81 * np.sin(t), 0.0], [0.0, 0.0, 0.0]])
+    start = eq[None] + np.random.RandomState(0).normal(0.0, 0.05, size=(n, 3, 3))
+    out = {"config": f"triatomic Morse+bend PES, {n} walkers, dt=5, {T} steps, continuous weights "
+                     f"(lower threshold 0.01, exact replay of the sequential split loop), descendant "
+                     f"weighting every 500 steps over 50 steps"}
+    for prec in ('fp32', 'fp64'):
+        sim = pv.DMC_Sim(num_walkers=n, num_timesteps=T, atoms=['H', 'H', 'O'], masses=[1837.15, 1837.15, 29156.95],
+                         delta_t=5, potential=pv.MorseBendTriatomic(), start_structures=start,
+                         weighting='continuous', cont_wt_thresh=0.01, equil_steps=500, wfn_every=500,
+                         desc_wt_steps=50, chkpt_every=10 ** 9, precision=prec, seed=4, write_files=False)
+        wall, _ = timed_propagate(sim)
+        nb, how = sim.trajectory_counts(0, T)
+        z, se = zpe_of(sim._vref_vs_tau)
+        lib = N.load()
+        avg = (ctypes.c_float * 8)()
+        nk = ctypes.c_int32(0)
+        N.check(lib.pvd_sim_run_timed(sim._h, 200, N.stream_handle(), avg, ctypes.byref(nk)))
+        sim._raise_if_failed()
+        kus = {nm: float(avg[k]) * 1e3 for k, nm in enumerate(['k_cont', 'k_cont_replay', 'k_cont_copy'][:nk.value])}
+        bpw = 88 if prec == 'fp32' else 160  # coordinates read + written, weight read + written
+        ach = bpw * n / (kus['k_cont'] * 1e-6) / 1e9
+        out[prec] = {"value": n * T / wall, "unit": "walker-steps/s", "ms_per_step": wall * 1e3 / T,
+                     "zpe_cm": z, "zpe_se_cm": se, "splits_per_step": float(np.mean(nb)),
+                     "replay_paths": {str(int(k)): int(v) for k, v in zip(*np.unique(how, return_counts=True))},
+                     "wfns": len(sim.wfns), "step_kernels_us": kus,
+                     "k_cont_roofline": {"bound": "hbm", "achieved": ach, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                                         "frac": ach / HBM_PEAK_GBS, "bytes_per_walker_step": bpw}}
+        log(f'[bench] continuous {prec}: {n * T / wall:.4g} walker-steps/s, {wall * 1e6 / T:.1f} us/step, '
+            f'ZPE {z:.2f} +- {se:.2f}, kernels {kus}')
+        del sim
+        torch.cuda.empty_cache()
+    return out
+
+
 def zpe_study(n, T, rates, model):
     """ZPE runs of T steps at n walkers: plain and importance-sampled DMC in FP32, their
     FP64 runs (the oracles the FP32 forms are judged against), and the surrogate form.
@@ -259,6 +308,7 @@ def main():
     ap.add_argument('--no-variants', action='store_true')
     ap.add_argument('--event-steps', type=int, default=200)
     ap.add_argument('--copy-only', action='store_true', help='only the 1 GiB copy (counter calibration)')
+    ap.add_argument('--continuous-only', action='store_true', help='only the continuous-weighting workload')
     ap.add_argument('--no-surrogate', action='store_true')
     ap.add_argument('--epochs', type=int, default=200)
     ap.add_argument('--train-geoms', type=int, default=2_000_000)
@@ -273,6 +323,9 @@ def main():
     assert torch.cuda.is_available(), 'bench.py needs the GPU'
     if args.copy_only:
         print(json.dumps({"copy_GBs": copy_bandwidth()}), flush=True)
+        return
+    if args.continuous_only:
+        print(json.dumps({"continuous": continuous_variant()}), flush=True)
         return
     n = args.walkers
     K, W = args.steps, args.warmup
@@ -322,6 +375,8 @@ def main():
         log(f'[bench] IS: {ws2 / wall2:.4g} walker-steps/s')
         del sim
         torch.cuda.empty_cache()
+        log('[bench] continuous-weighting variant')
+        variants['continuous'] = continuous_variant()
 
     if not args.no_variants and not args.no_surrogate:
         import pyvibdmc_amd as pv

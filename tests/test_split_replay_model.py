@@ -1,0 +1,142 @@
+"""A model of the GPU's candidate split replay against the oracle's sequential split loop.
+
+The continuous-weighting replay on the GPU (k_cont_replay, candidate path) does not run
+the reference's loop (pyvibdmc.py:344-356: take the heaviest walker, first slot on ties,
+halve it, copy it with the other half into the next killed slot).  It computes the loop's
+outcome in closed form from the walkers of weight >= T0 (T0 >= the lower threshold):
+
+* a walker of weight w = m 2^e split d levels deep is 2^d copies of weight w 2^-d;
+* the octave [2^E, 2^(E+1)) holds, for every candidate with e >= E, its 2^(e-E) entries of
+  the single value m 2^E; the loop takes octaves from the top, families of one octave in
+  decreasing mantissa, entries of one family in slot order;
+* so a scan of family sizes per octave assigns the kill indices, until K are used, and
+  only the family where the kills run out needs its slots ranked.
+
+The model below states exactly those steps in Python (the same decisions the kernel makes,
+including when it declines: equal mantissas meeting at or above the last octave used, or
+too few entries at or above T0).  Where it does not decline it must reproduce the oracle's
+loop bit for bit: final weights and the source of every slot.
+"""
+import math
+
+import numpy as np
+import pytest
+
+from oracle.dmc import OracleDMC
+
+MANT = (1 << 52) - 1
+
+
+def _bits(x):
+    return int(np.float64(x).view(np.uint64))
+
+
+def _exp(x):
+    return ((_bits(x) >> 52) & 0x7FF) - 1023
+
+
+def oracle_split(w, thresh):
+    """The oracle's _branch (pyvibdmc.py:340-356) on weights alone; returns the final
+    weights and each slot's source walker."""
+    class S:
+        pass
+    s = S()
+    s.cont_wts = np.array(w, dtype=np.float64)
+    s.coords = np.arange(len(w), dtype=np.float64)[:, None, None] * np.ones((1, 1, 1))
+    s.V = np.zeros(len(w))
+    s._desc_wt = True
+    s._who_from = np.arange(len(w))
+    s.trial = None
+    OracleDMC._branch(s, np.where(s.cont_wts < thresh)[0])
+    return s.cont_wts, s._who_from
+
+
+def candidate_replay(w, thresh, T0):
+    """The closed-form replay; None where the GPU declines (and runs its general replay)."""
+    n = len(w)
+    kl = np.where(w < thresh)[0]
+    K = len(kl)
+    assert T0 >= thresh
+    cand = [i for i in range(n) if w[i] >= T0]
+    if K == 0 or not cand:
+        return None
+    # families by decreasing mantissa (ties in any order: they make the model decline)
+    cs = sorted(cand, key=lambda i: -(_bits(w[i]) & MANT))
+    ms = [_bits(w[o]) & MANT for o in cs]
+    es = [_exp(w[o]) for o in cs]
+    mT, eT = _bits(T0) & MANT, _exp(T0)
+    C = len(cs)
+    asg = np.full(K, -1)
+    base, Es = 0, None
+    for E in range(max(es), eT - 1, -1):
+        size = [(1 << (es[j] - E)) if es[j] >= E and (E > eT or ms[j] >= mT) else 0 for j in range(C)]
+        start = base + np.concatenate([[0], np.cumsum(size)[:-1]])
+        tot = int(sum(size))
+        for j in range(C):
+            asg[start[j]:min(start[j] + size[j], K)] = j
+        if base + tot >= K:
+            Es = E
+            p = max(j for j in range(C) if size[j] > 0 and start[j] < K)
+            startp = int(start[p])
+            break
+        base += tot
+    if Es is None:
+        return None  # fewer than K entries at or above T0
+    if any(ms[j] == ms[j + 1] and min(es[j], es[j + 1]) >= Es for j in range(C - 1)):
+        return None  # two families tie in an octave that is used
+    r = K - startp
+    D = [0 if es[j] < Es else es[j] - Es + (1 if j < p else 0) for j in range(C)]
+    wn, src = np.array(w, dtype=np.float64), np.arange(n)
+    for t in range(K):
+        j = asg[t]
+        src[kl[t]] = cs[j]
+        if j != p:
+            wn[kl[t]] = math.ldexp(w[cs[j]], -D[j])
+        elif t >= startp:
+            wn[kl[t]] = math.ldexp(w[cs[p]], -(D[p] + 1))
+    for j in range(C):
+        if j != p and D[j] > 0:
+            wn[cs[j]] = math.ldexp(w[cs[j]], -D[j])
+    members = sorted([cs[p]] + [kl[t] for t in range(startp) if asg[t] == p])
+    assert len(members) == 1 << D[p]
+    for q, sl in enumerate(members):
+        wn[sl] = math.ldexp(w[cs[p]], -(D[p] + (1 if q < r else 0)))
+    return wn, src
+
+
+def _cases(seed, count):
+    rs = np.random.RandomState(seed)
+    for i in range(count):
+        n = int(rs.randint(5, 300))
+        kind = i % 5
+        if kind == 0:
+            w = np.exp(rs.normal(0, 1.0, n))
+        elif kind == 1:
+            w = np.exp(rs.normal(0, 2.5, n))  # deep splits
+        elif kind == 2:
+            # exact powers of two apart, and equal weights: ties across families
+            w = rs.choice([0.5, 1.0, 2.0, 0.25, 3.0, 0.001], n) * np.where(rs.rand(n) < 0.5, 1.0, np.exp(rs.normal(0, 1e-3, n)))
+        elif kind == 3:
+            w = np.exp(rs.normal(0, 1.0, n))
+            w[rs.rand(n) < 0.2] = 0.005
+        else:
+            w = np.exp(rs.uniform(-4.8, 2.0, n))  # the log-uniform start of the GPU tests
+        thresh = float(np.quantile(w, rs.uniform(0.01, 0.3)))
+        T0 = max(thresh, float(np.quantile(w, rs.uniform(0.3, 0.99))))
+        yield w, thresh, T0
+
+
+@pytest.mark.parametrize('seed', [1, 2])
+def test_candidate_replay_reproduces_the_sequential_loop(seed):
+    served = declined = 0
+    for w, thresh, T0 in _cases(seed, 700):
+        got = candidate_replay(w, thresh, T0)
+        if got is None:
+            declined += 1
+            continue
+        served += 1
+        ref_w, ref_src = oracle_split(w, thresh)
+        np.testing.assert_array_equal(got[0], ref_w)
+        np.testing.assert_array_equal(got[1], ref_src)
+    # the model must serve most cases (it declines only on ties and on too high a T0)
+    assert served > 300, (served, declined)
