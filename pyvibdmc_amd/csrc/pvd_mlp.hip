@@ -703,6 +703,141 @@ __global__ __launch_bounds__(64 * E_WAVES, 2) void k_mlp_energy(MlpNet M, const 
   }
 }
 
+// Inference on walkers when the two hidden-layer packs do not fit LDS together (H = 256:
+// 128 KB each).  Persistent workgroups of E_WAVES waves keep layer 0's pack, the biases and
+// the output weights in LDS for the whole launch, and stage the packs of layers 1 and 2 in
+// turn through one slot, once per chunk of E_WAVES x 16 x NTS walkers (the packs come from
+// L2; the chunk's activations stay in registers between the layers).
+template <int H>
+constexpr size_t staged_lds_bytes() {
+  return (size_t)((H / 16) * 64 + (H / 16) * (H / 32) * 64) * 16 + (size_t)(4 * H + 4) * 4;
+}
+
+template <typename R, int NP, int H, int NTS>
+__global__ __launch_bounds__(64 * E_WAVES, 1) void k_mlp_energy_st(MlpNet M, const R* planes, long long stride,
+                                                                  const int* n_dev, long long n_fix, double* V) {
+  extern __shared__ bf16x8 lds[];
+  constexpr int MT = H / 16, KC = H / 32, NTH = 64 * E_WAVES;
+  bf16x8* L0 = lds;
+  bf16x8* LW = L0 + MT * 64;
+  float* lb = (float*)(LW + MT * KC * 64);  // b0 | b1 | b2 | w3 | b3
+  for (int q = threadIdx.x; q < MT * 64; q += NTH) L0[q] = M.Wf[0][q];
+  for (int q = threadIdx.x; q < H; q += NTH) {
+    lb[q] = M.b[0][q];
+    lb[H + q] = M.b[1][q];
+    lb[2 * H + q] = M.b[2][q];
+    lb[3 * H + q] = M.w3[q];
+  }
+  if (threadIdx.x == 0) lb[4 * H] = M.b3[0];
+  __syncthreads();
+  const int n = n_dev ? *n_dev : (int)n_fix;
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const int g = lane >> 4, col = lane & 15;
+  constexpr long long CHUNK = (long long)E_WAVES * 16 * NTS;
+  // the loop bounds depend on the workgroup only: every wave meets every barrier
+  for (long long c0 = (long long)blockIdx.x * CHUNK; c0 < n; c0 += (long long)gridDim.x * CHUNK) {
+    asm volatile("" ::: "memory");
+    const long long base = c0 + (long long)wave * 16 * NTS;
+    constexpr int NPR = NP / 3 * (NP / 3 - 1) / 2;
+    bf16x8 bin0[NTS][1];
+    bf16x8 bin[NTS][KC];
+    bool fbk[NTS];
+    float qb[NTS];
+#pragma unroll
+    for (int t = 0; t < NTS; ++t) {
+      const long long w = base + t * 16 + col;
+      const bool ok = w < n;
+      float f[MLP_MAXP];
+      R x[NP];
+#pragma unroll
+      for (int p = 0; p < NP; ++p) x[p] = ok ? planes[p * stride + w] : R(0);
+      mlp_descriptor<R, NP>(M, x, f);
+      bool out = false;
+      if (M.use_range) {
+#pragma unroll
+        for (int k = 0; k < NPR; ++k) out |= !(f[k] >= M.lo[k] && f[k] <= M.hi[k]);
+      }
+      fbk[t] = out;
+      bin0[t][0] = input_chunk<NPR>(M, f, g, ok);
+      qb[t] = 0.0f;
+      if (g == 0) {
+        float xn[NPR];
+#pragma unroll
+        for (int k = 0; k < NPR; ++k) xn[k] = (f[k] - M.xmean[k]) * M.xinv[k];
+        qb[t] = quad_baseline<NPR>(M, xn);
+      }
+    }
+    f32x4 acc[NTS][MT];
+    float part[NTS];
+    // layer 0 from the resident pack
+#pragma unroll
+    for (int m = 0; m < MT; ++m) {
+      const f32x4 bias = *(const f32x4*)(lb + 16 * m + 4 * g);
+      const bf16x8 a = L0[m * 64 + lane];
+#pragma unroll
+      for (int t = 0; t < NTS; ++t) acc[t][m] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, bin0[t][0], bias, 0, 0, 0);
+    }
+#pragma unroll
+    for (int t = 0; t < NTS; ++t)
+#pragma unroll
+      for (int m = 0; m < MT; ++m)
+#pragma unroll
+        for (int i = 0; i < 4; ++i) bin[t][m >> 1][(m & 1) * 4 + i] = (__bf16)silu_f(acc[t][m][i]);
+    // layers 1 and 2 through the staging slot
+#pragma unroll
+    for (int layer = 1; layer < 3; ++layer) {
+      __syncthreads();  // every wave is done with the slot
+      const bf16x8* src = M.Wf[layer];
+      for (int q = threadIdx.x; q < MT * KC * 64; q += NTH) LW[q] = src[q];
+      __syncthreads();
+      const float* bl = lb + layer * H;
+#pragma unroll
+      for (int m = 0; m < MT; ++m) {
+        const f32x4 bias = *(const f32x4*)(bl + 16 * m + 4 * g);
+#pragma unroll
+        for (int c = 0; c < KC; ++c) {
+          const bf16x8 a = LW[(m * KC + c) * 64 + lane];
+#pragma unroll
+          for (int t = 0; t < NTS; ++t)
+            acc[t][m] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, bin[t][c], c == 0 ? bias : acc[t][m], 0, 0, 0);
+        }
+      }
+      if (layer == 1) {
+#pragma unroll
+        for (int t = 0; t < NTS; ++t)
+#pragma unroll
+          for (int m = 0; m < MT; ++m)
+#pragma unroll
+            for (int i = 0; i < 4; ++i) bin[t][m >> 1][(m & 1) * 4 + i] = (__bf16)silu_f(acc[t][m][i]);
+      } else {
+#pragma unroll
+        for (int t = 0; t < NTS; ++t) {
+          float s = 0.0f;
+#pragma unroll
+          for (int m = 0; m < MT; ++m) {
+            const f32x4 w3 = *(const f32x4*)(lb + 3 * H + 16 * m + 4 * g);
+#pragma unroll
+            for (int i = 0; i < 4; ++i) s += w3[i] * silu_f(acc[t][m][i]);
+          }
+          part[t] = s;
+        }
+      }
+    }
+    const float b3 = lb[4 * H];
+#pragma unroll
+    for (int t = 0; t < NTS; ++t) {
+      float s = part[t];
+      s += __shfl_xor(s, 16, 64);
+      s += __shfl_xor(s, 32, 64);
+      const long long w = base + t * 16 + col;
+      if (g == 0 && w < n) {
+        const float y = s + b3;
+        V[w] = fbk[t] ? __longlong_as_double((long long)FALLBACK_MARK) : (double)(y * M.ystd + M.ymean + qb[t]);
+      }
+    }
+  }
+}
+
 // analytic potential for the walkers the surrogate marked as outside its training range
 template <typename R, int NP>
 __global__ __launch_bounds__(256) void k_mlp_fallback(const R* planes, long long stride, const int* n_dev,
@@ -790,16 +925,27 @@ int launch_energy(const MlpNet& N, const void* planes, long long stride, const i
       MHIP(hipGetLastError());
     }
     return 0;
-  }
-  const long long per_block = 4ll * 16 * NT_FWD;
-  const unsigned grid = (unsigned)((n_max + per_block - 1) / per_block);
-  const size_t lds = (size_t)(H / 16) * (H / 32) * 64 * 16;
-  auto k = k_mlp_forward<R, NP, H, NT_FWD, 0>;
+  } else {
+  // H = 256: persistent, one workgroup per CU (LDS-bound), layers 1 and 2 staged per chunk
+  int dev = 0, cus = 256;
+  MHIP(hipGetDevice(&dev));
+  MHIP(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev));
+  constexpr int NTS = 1;  // one 16-walker tile per wave: the accumulators and activations fit registers
+  const long long chunks = (n_max + (long long)E_WAVES * 16 * NTS - 1) / ((long long)E_WAVES * 16 * NTS);
+  const unsigned grid = (unsigned)std::max(1ll, std::min(chunks, (long long)cus));
+  const size_t lds = staged_lds_bytes<H>();
+  auto k = k_mlp_energy_st<R, NP, H, NTS>;
   MHIP(hipFuncSetAttribute((const void*)k, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
-  hipLaunchKernelGGL(k, dim3(grid), dim3(256), lds, st, N, (const R*)planes, stride, n_dev, n_max, V, fb,
-                     nullptr, nullptr, nullptr);
+  hipLaunchKernelGGL(k, dim3(grid), dim3(64 * E_WAVES), lds, st, N, (const R*)planes, stride, n_dev, n_max, V);
   MHIP(hipGetLastError());
+  if (N.use_range) {
+    const unsigned gf = (unsigned)std::max(1ll, std::min((n_max + 255) / 256, 4ll * cus));
+    hipLaunchKernelGGL((k_mlp_fallback<R, NP>), dim3(gf), dim3(256), 0, st, (const R*)planes, stride, n_dev, n_max,
+                       V, fb);
+    MHIP(hipGetLastError());
+  }
   return 0;
+  }
 }
 
 }  // namespace
