@@ -124,3 +124,37 @@ def test_continuous_large_population_matches_oracle(n, steps):
     coords, wts = sim._get_walkers()
     np.testing.assert_array_equal(coords, orc.coords)
     np.testing.assert_allclose(wts, orc.cont_wts, rtol=1e-12, atol=0)
+
+
+def test_continuous_replay_with_a_cut_inside_the_list_matches_oracle():
+    """A tenth of the walkers start in one dense cluster of near-equal heavy weights (5.0,
+    spread 1e-6) and 3 % below the threshold.  After the first step (interval replay) the
+    candidate list holds more walkers than the replay sorts, so the replay cuts inside it
+    (path 5) before the later steps take the whole list.  Everything matches the oracle's
+    sequential split loop."""
+    import pyvibdmc_amd as pv
+    from oracle import pes as P
+    from oracle.dmc import LegacyDraws, OracleDMC
+    n, steps = 100_000, 8
+    rs = np.random.RandomState(30)
+    start = P.water_eq()[None] + rs.normal(0, 0.05, (n, 3, 3))
+    w0 = np.exp(rs.uniform(-3.0, 0.5, n))
+    sel = rs.rand(n)
+    heavy = sel < 0.10
+    w0[heavy] = 5.0 * (1.0 + 1e-6 * rs.normal(size=heavy.sum()))
+    w0[(sel >= 0.10) & (sel < 0.13)] = 0.005
+    kw = dict(masses=[1837.15, 1837.15, 29156.95], start_structures=start, num_walkers=n, num_timesteps=steps,
+              delta_t=5, equil_steps=10 ** 9, wfn_every=10 ** 9, weighting='continuous', cont_wt_thresh=0.01,
+              start_cont_wts=w0)
+    orc = OracleDMC(potential=P.morse_bend, draws=LegacyDraws(9), **kw)
+    orc.propagate()
+    sim = pv.DMC_Sim(atoms=['H', 'H', 'O'], potential=pv.MorseBendTriatomic(), precision='fp64',
+                     rng=LegacyDraws(9), write_files=False, chkpt_every=10 ** 9, **kw)
+    sim.propagate()
+    nb, how = sim.trajectory_counts(0, steps)
+    np.testing.assert_array_equal(nb, orc.rec['n_branched'])
+    assert how[0] == 2 and 5 in how.tolist(), how
+    np.testing.assert_allclose(sim._vref_vs_tau, orc._vref_vs_tau, rtol=0, atol=1e-10)
+    coords, wts = sim._get_walkers()
+    np.testing.assert_array_equal(coords, orc.coords)
+    np.testing.assert_allclose(wts, orc.cont_wts, rtol=1e-12, atol=0)
