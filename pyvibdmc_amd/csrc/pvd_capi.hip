@@ -371,12 +371,20 @@ StepArgs<R> make_args(pvd_sim* s) {
 
 // The compiled model specialisation for a configuration: MS_PES_WATER for the [H, H, O]
 // Morse+bend surface in 3 dimensions, plus MS_TRIAL_WATER when importance sampling uses the
-// matching two-bond / one-angle Gaussian trial.  A water trial on another surface keeps
-// the runtime parameters.
+// matching two-bond / one-angle Gaussian trial; MS_PES_CLUSTER / MS_TRIAL_CLUSTER for the
+// [O, H, H, O, H, H] cluster surface and its per-monomer trial.  Other layouts keep the
+// runtime parameters.
 int model_spec(const pvd_config& c, int np) {
-  if (np != 9 || c.n_atoms != 3 || c.n_dim != 3) return MS_ANY;
   const pvd_pes_desc& p = c.pes;
   const pvd_trial_desc& t = c.trial;
+  if (np == 18 && c.n_atoms == 6 && c.n_dim == 3) {
+    const bool pc = p.kind == PES_CLUSTER6;
+    const bool tc = t.kind == TRIAL_GAUSS && t.nb == 4 && t.na == 2 && t.bi[0] == 1 && t.bj[0] == 0 &&
+                    t.bi[1] == 2 && t.bj[1] == 0 && t.bi[2] == 4 && t.bj[2] == 3 && t.bi[3] == 5 && t.bj[3] == 3 &&
+                    t.ai[0] == 1 && t.aj[0] == 0 && t.ak[0] == 2 && t.ai[1] == 4 && t.aj[1] == 3 && t.ak[1] == 5;
+    return (pc ? MS_PES_CLUSTER : MS_ANY) | (tc ? MS_TRIAL_CLUSTER : MS_ANY);
+  }
+  if (np != 9 || c.n_atoms != 3 || c.n_dim != 3) return MS_ANY;
   const bool pw = p.kind == PES_MORSE_BEND && p.idx[0] == 0 && p.idx[1] == 1 && p.idx[2] == 2;
   const bool tw = t.kind == TRIAL_GAUSS && t.nb == 2 && t.na == 1 && t.bi[0] == 0 && t.bj[0] == 2 &&
                   t.bi[1] == 1 && t.bj[1] == 2 && t.ai[0] == 0 && t.aj[0] == 2 && t.ak[0] == 1;
@@ -393,9 +401,24 @@ int model_spec(const pvd_config& c, int np) {
     default: return fail(E_BAD, "fused kernels support 1, 3, 9 or 18 components per walker"); \
   }
 
-// component count and model specialisation (the specialisations exist for 9 components)
+// component count and model specialisation (the specialisations exist for 9 and 18
+// components)
 #define PVD_STEP_SWITCH(sim, ...)                                               \
-  PVD_NP_SWITCH((sim)->np, if constexpr (NP == 9) {                             \
+  PVD_NP_SWITCH((sim)->np, if constexpr (NP == 18) {                            \
+    if ((sim)->mspec == (MS_PES_CLUSTER | MS_TRIAL_CLUSTER)) {                  \
+      constexpr int MS = MS_PES_CLUSTER | MS_TRIAL_CLUSTER;                     \
+      __VA_ARGS__;                                                              \
+    } else if ((sim)->mspec == MS_TRIAL_CLUSTER) {                              \
+      constexpr int MS = MS_TRIAL_CLUSTER;                                      \
+      __VA_ARGS__;                                                              \
+    } else if ((sim)->mspec == MS_PES_CLUSTER) {                                \
+      constexpr int MS = MS_PES_CLUSTER;                                        \
+      __VA_ARGS__;                                                              \
+    } else {                                                                    \
+      constexpr int MS = MS_ANY;                                                \
+      __VA_ARGS__;                                                              \
+    }                                                                           \
+  } else if constexpr (NP == 9) {                                               \
     if ((sim)->mspec == (MS_PES_WATER | MS_TRIAL_WATER)) {                      \
       constexpr int MS = MS_PES_WATER | MS_TRIAL_WATER;                         \
       __VA_ARGS__;                                                              \
@@ -461,14 +484,14 @@ int step_kernels(pvd_sim* s, StepArgs<R>& A, Launch* L, int& nl) {
     if (s->is) {
       PVD_STEP_SWITCH(s, add_disc((const void*)k_discrete<R, NP, INJ, true, false, MS>, disc_wpt<R, NP, false, true>()));
     } else {
-      PVD_STEP_SWITCH(s, add_disc((const void*)k_discrete<R, NP, INJ, false, false, (MS & MS_PES_WATER)>,
+      PVD_STEP_SWITCH(s, add_disc((const void*)k_discrete<R, NP, INJ, false, false, (MS & MS_PES_ANY_SPEC)>,
                                   disc_wpt<R, NP, false>()));
     }
   } else {
     if (s->is) {
       PVD_ADD(k_cont<R, NP, INJ, true, false, MS>);
     } else {
-      PVD_ADD(k_cont<R, NP, INJ, false, false, (MS & MS_PES_WATER)>);
+      PVD_ADD(k_cont<R, NP, INJ, false, false, (MS & MS_PES_ANY_SPEC)>);
     }
     add((const void*)k_cont_replay<R>, 1, RP_NT);
     add((const void*)k_cont_copy<R>, KILL_CAP / TILE, TILE);
@@ -669,9 +692,9 @@ int do_move(pvd_sim* s, const double* disp, const double* um, hipStream_t st) {
   const unsigned g = nblk(s->cap, TILE);
   if (s->is) {
     if (disp) {
-      PVD_STEP_SWITCH(s, hipLaunchKernelGGL((k_move<R, NP, true, true, (MS & MS_TRIAL_WATER)>), dim3(g), dim3(TILE), 0, st, A));
+      PVD_STEP_SWITCH(s, hipLaunchKernelGGL((k_move<R, NP, true, true, (MS & MS_TRIAL_ANY_SPEC)>), dim3(g), dim3(TILE), 0, st, A));
     } else {
-      PVD_STEP_SWITCH(s, hipLaunchKernelGGL((k_move<R, NP, false, true, (MS & MS_TRIAL_WATER)>), dim3(g), dim3(TILE), 0, st, A));
+      PVD_STEP_SWITCH(s, hipLaunchKernelGGL((k_move<R, NP, false, true, (MS & MS_TRIAL_ANY_SPEC)>), dim3(g), dim3(TILE), 0, st, A));
     }
   } else {
     if (disp) hipLaunchKernelGGL((k_move<R, 1, true, false>), dim3(g), dim3(TILE), 0, st, A);

@@ -331,6 +331,8 @@ PVD_DEV R is_move_fast(const StepArgs<R>& A, const R* disp, double u, R* x, bool
   R lx, ly, lapx, lapy, gx[NP], gy[NP], y[NP];
   if constexpr ((MS & MS_TRIAL_WATER) != 0 && NP == 9) {
     trial_water_fast<R>(A.trial, x, (R)A.inv_m[0], (R)A.inv_m[6], lx, gx, lapx);
+  } else if constexpr ((MS & MS_TRIAL_CLUSTER) != 0 && NP == 18) {
+    trial_cluster_fast<R>(A.trial, x, (R)A.inv_m[3], (R)A.inv_m[0], lx, gx, lapx);
   } else {
     trial_gauss_fast<R, NP>(A.trial, A.inv_m, x, lx, gx, lapx);
   }
@@ -338,6 +340,8 @@ PVD_DEV R is_move_fast(const StepArgs<R>& A, const R* disp, double u, R* x, bool
   for (int p = 0; p < NP; ++p) y[p] = (x[p] + disp[p]) + ((R)A.inv_m[p] * gx[p]) * dt;
   if constexpr ((MS & MS_TRIAL_WATER) != 0 && NP == 9) {
     trial_water_fast<R>(A.trial, y, (R)A.inv_m[0], (R)A.inv_m[6], ly, gy, lapy);
+  } else if constexpr ((MS & MS_TRIAL_CLUSTER) != 0 && NP == 18) {
+    trial_cluster_fast<R>(A.trial, y, (R)A.inv_m[3], (R)A.inv_m[0], ly, gy, lapy);
   } else {
     trial_gauss_fast<R, NP>(A.trial, A.inv_m, y, ly, gy, lapy);
   }
@@ -366,7 +370,7 @@ PVD_DEV R is_move_fast(const StepArgs<R>& A, const R* disp, double u, R* x, bool
 template <typename R, int NP, bool INJ, int MS = MS_ANY>
 PVD_DEV R is_move(const StepArgs<R>& A, const R* disp, double u, R* x, bool& acc) {
   if constexpr (sizeof(R) == 4 && NP % 3 == 0 && NP >= 6) {
-    if ((MS & MS_TRIAL_WATER) != 0 || A.trial.kind == TRIAL_GAUSS) return is_move_fast<R, NP, MS>(A, disp, u, x, acc);
+    if ((MS & MS_TRIAL_ANY_SPEC) != 0 || A.trial.kind == TRIAL_GAUSS) return is_move_fast<R, NP, MS>(A, disp, u, x, acc);
   }
   TrialOut<R, NP> tx, ty;
   trial_eval_s<R, NP, MS>(A.trial, x, tx);

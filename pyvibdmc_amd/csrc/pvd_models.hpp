@@ -143,6 +143,26 @@ PVD_DEV void gather3(const R* x, const int* idx, R* g) {
 }
 
 // Evaluate a device PES.  NP is the number of Cartesian components per walker.
+// The [O, H, H, O, H, H] cluster: two Morse+bend monomers, an O-O Morse term and the
+// Coulomb energy of the inter-monomer atom pairs.
+template <typename R>
+PVD_DEV R cluster6_v(const PesParams& P, const R* x) {
+  const R De = R(P.p[0]), a = R(P.p[1]), re = R(P.p[2]), hk = R(0.5 * P.p[3]), th0 = R(P.p[4]);
+  R v = morse_bend_v<R>(x, 3, 6, 0, De, a, re, hk, th0) + morse_bend_v<R>(x, 12, 15, 9, De, a, re, hk, th0);
+  const R roo = norm3(x[0] - x[9], x[1] - x[10], x[2] - x[11]);
+  const R m = R(1) - pvd_exp<R>(-R(P.p[6]) * (roo - R(P.p[7])));
+  v = v + R(P.p[5]) * (m * m);
+  const R q[3] = {R(P.p[8]), R(P.p[9]), R(P.p[9])};
+#pragma unroll
+  for (int i = 0; i < 3; ++i)
+#pragma unroll
+    for (int j = 0; j < 3; ++j) {
+      const R r = norm3(x[3 * i] - x[9 + 3 * j], x[3 * i + 1] - x[10 + 3 * j], x[3 * i + 2] - x[11 + 3 * j]);
+      v = v + (q[i] * q[j]) / r;
+    }
+  return v;
+}
+
 template <typename R, int NP>
 PVD_DEV R pes_eval(const PesParams& P, const R* x) {
   switch (P.kind) {
@@ -165,22 +185,7 @@ PVD_DEV R pes_eval(const PesParams& P, const R* x) {
       }
       break;
     case PES_CLUSTER6:
-      if constexpr (NP == 18) {
-        const R De = R(P.p[0]), a = R(P.p[1]), re = R(P.p[2]), hk = R(0.5 * P.p[3]), th0 = R(P.p[4]);
-        R v = morse_bend_v<R>(x, 3, 6, 0, De, a, re, hk, th0) + morse_bend_v<R>(x, 12, 15, 9, De, a, re, hk, th0);
-        const R roo = norm3(x[0] - x[9], x[1] - x[10], x[2] - x[11]);
-        const R m = R(1) - pvd_exp<R>(-R(P.p[6]) * (roo - R(P.p[7])));
-        v = v + R(P.p[5]) * (m * m);
-        const R q[3] = {R(P.p[8]), R(P.p[9]), R(P.p[9])};
-#pragma unroll
-        for (int i = 0; i < 3; ++i)
-#pragma unroll
-          for (int j = 0; j < 3; ++j) {
-            const R r = norm3(x[3 * i] - x[9 + 3 * j], x[3 * i + 1] - x[10 + 3 * j], x[3 * i + 2] - x[11 + 3 * j]);
-            v = v + (q[i] * q[j]) / r;
-          }
-        return v;
-      }
+      if constexpr (NP == 18) return cluster6_v<R>(P, x);
       break;
     default:
       break;
@@ -440,14 +445,21 @@ PVD_DEV void trial_eval(const TrialParams& T, const R* x, TrialOut<R, NP>& o) {
 
 // Compile-time model specialisations of the step kernels.  The host selects them when the
 // model matches exactly, so the kernels carry no runtime index selects for that layout:
-//   MS_PES_WATER    PES_MORSE_BEND with atoms (h1, h2, o) = (0, 1, 2), 3 atoms x 3 dims
-//   MS_TRIAL_WATER  TRIAL_GAUSS with bonds (0-2), (1-2) and the angle 0-2-1, same layout
-enum ModelSpec : int { MS_ANY = 0, MS_PES_WATER = 1, MS_TRIAL_WATER = 2 };
+//   MS_PES_WATER      PES_MORSE_BEND with atoms (h1, h2, o) = (0, 1, 2), 3 atoms x 3 dims
+//   MS_TRIAL_WATER    TRIAL_GAUSS with bonds (0-2), (1-2) and the angle 0-2-1, same layout
+//   MS_PES_CLUSTER    PES_CLUSTER6, atoms [O, H, H, O, H, H] in 3 dims
+//   MS_TRIAL_CLUSTER  TRIAL_GAUSS with bonds (1-0), (2-0), (4-3), (5-3) and the angles
+//                     1-0-2, 4-3-5, same layout
+enum ModelSpec : int { MS_ANY = 0, MS_PES_WATER = 1, MS_TRIAL_WATER = 2, MS_PES_CLUSTER = 4, MS_TRIAL_CLUSTER = 8 };
+constexpr int MS_PES_ANY_SPEC = MS_PES_WATER | MS_PES_CLUSTER;
+constexpr int MS_TRIAL_ANY_SPEC = MS_TRIAL_WATER | MS_TRIAL_CLUSTER;
 
 template <typename R, int NP, int MS>
 PVD_DEV R pes_eval_s(const PesParams& P, const R* x) {
   if constexpr ((MS & MS_PES_WATER) != 0 && NP == 9) {
     return morse_bend_v<R>(x, 0, 3, 6, R(P.p[0]), R(P.p[1]), R(P.p[2]), R(0.5 * P.p[3]), R(P.p[4]));
+  } else if constexpr ((MS & MS_PES_CLUSTER) != 0 && NP == 18) {
+    return cluster6_v<R>(P, x);
   } else {
     return pes_eval<R, NP>(P, x);
   }
@@ -592,6 +604,63 @@ PVD_DEV void trial_water_fast(const TrialParams& T, const R* x, R im_h, R im_o, 
   const R la_k = -at * (irv2 + dq * cot * irv2);
   const R la_j = -at * ((iru2 + irv2 - R(2) * cth * iru * irv) + dq * (cot * (iru2 + irv2) - R(2) * iru * irv * isth));
   lap = im_h * (lb1 + lb2 + la_i + la_k) + im_o * (lb1 + lb2 + la_j);
+}
+
+// One monomer of a Gaussian-product trial in the closed form of trial_water_fast: bonds
+// h1-o and h2-o (parameters KB, KB + 1) and the angle h1-o-h2 (parameter KA), atoms at the
+// compile-time component offsets H1, H2, O.  Adds to q2 = sum alpha (q - q0)^2, to the
+// gradient g of ln psi and to the mass-weighted Laplacian lap.
+template <typename R, int H1, int H2, int O, int KB, int KA>
+PVD_DEV void trial_mono_acc(const TrialParams& T, const R* x, R im_h, R im_o, R& q2, R* g, R& lap) {
+  R u[3], v[3];
+#pragma unroll
+  for (int c = 0; c < 3; ++c) {
+    u[c] = x[H1 + c] - x[O + c];
+    v[c] = x[H2 + c] - x[O + c];
+  }
+  const R ru = norm3(u[0], u[1], u[2]), rv = norm3(v[0], v[1], v[2]);
+  const R iru = R(1) / ru, irv = R(1) / rv;
+  const R a1 = R(T.balpha[KB]), a2 = R(T.balpha[KB + 1]), at = R(T.aalpha[KA]);
+  const R d1 = ru - R(T.br0[KB]), d2 = rv - R(T.br0[KB + 1]);
+  R uh[3], vh[3];
+#pragma unroll
+  for (int c = 0; c < 3; ++c) {
+    uh[c] = u[c] * iru;
+    vh[c] = v[c] * irv;
+  }
+  const R cth = clamp1(uh[0] * vh[0] + uh[1] * vh[1] + uh[2] * vh[2]);
+  const R sth = pvd_sqrt<R>(R(1) - cth * cth);
+  const R isth = R(1) / sth;
+  const R dq = pvd_acos<R>(cth) - R(T.ath0[KA]);
+  q2 += (a1 * (d1 * d1) + a2 * (d2 * d2)) + at * (dq * dq);
+  const R gb1 = -a1 * d1, gb2 = -a2 * d2, ga = -at * dq;
+#pragma unroll
+  for (int c = 0; c < 3; ++c) {
+    const R tu = -(vh[c] - cth * uh[c]) * (iru * isth);
+    const R tv = -(uh[c] - cth * vh[c]) * (irv * isth);
+    g[H1 + c] += gb1 * uh[c] + ga * tu;
+    g[H2 + c] += gb2 * vh[c] + ga * tv;
+    g[O + c] += -(gb1 * uh[c] + gb2 * vh[c]) - ga * (tu + tv);
+  }
+  const R iru2 = iru * iru, irv2 = irv * irv, cot = cth * isth;
+  const R lb1 = -a1 * (R(1) + d1 * R(2) * iru), lb2 = -a2 * (R(1) + d2 * R(2) * irv);
+  const R la_i = -at * (iru2 + dq * cot * iru2);
+  const R la_k = -at * (irv2 + dq * cot * irv2);
+  const R la_j = -at * ((iru2 + irv2 - R(2) * cth * iru * irv) + dq * (cot * (iru2 + irv2) - R(2) * iru * irv * isth));
+  lap += im_h * (lb1 + lb2 + la_i + la_k) + im_o * (lb1 + lb2 + la_j);
+}
+
+// The [O, H, H, O, H, H] cluster trial (bonds 1-0, 2-0, 4-3, 5-3; angles 1-0-2, 4-3-5) as
+// two monomers; im_h, im_o: 1/m of the hydrogens and of the oxygens.
+template <typename R>
+PVD_DEV void trial_cluster_fast(const TrialParams& T, const R* x, R im_h, R im_o, R& lnpsi, R* g, R& lap) {
+#pragma unroll
+  for (int p = 0; p < 18; ++p) g[p] = R(0);
+  R q2 = R(0);
+  lap = R(0);
+  trial_mono_acc<R, 3, 6, 0, 0, 0>(T, x, im_h, im_o, q2, g, lap);
+  trial_mono_acc<R, 12, 15, 9, 2, 1>(T, x, im_h, im_o, q2, g, lap);
+  lnpsi = R(T.c0) - R(0.5) * q2;
 }
 
 template <typename R, int NP, int MS>
