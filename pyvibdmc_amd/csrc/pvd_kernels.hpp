@@ -1030,7 +1030,9 @@ __global__ __launch_bounds__(TILE) void k_cont(StepArgs<R> A) {
 }
 
 // Exact replay of the sequential split loop (pyvibdmc.py:344-356) for all walkers below
-// the threshold, in one workgroup.
+// the threshold, in one workgroup (k_cont_replay).  It first tries the closed form from
+// the candidate lists (replay_candidates, below); the interval replay described here is
+// the general path for what that leaves (the first step, ties, lists that miss).
 //
 // The loop repeatedly takes the walker of largest weight (first index on ties), halves
 // it and gives the other half to the next walker on the kill list.  The chosen weights
