@@ -184,3 +184,25 @@ def test_surrogate_dmc_reproduces_analytic_zpe_and_beats_its_baseline():
     d_base = abs(z['baseline'] - z['analytic'])
     assert d_sur < 1.5, z
     assert d_base > 3 * d_sur and d_base > 4.0, z
+
+
+@pytest.mark.parametrize('hidden', [128, 256])
+def test_persistent_inference_over_many_chunks(hidden):
+    """300 007 walkers: each persistent workgroup loops over several chunks (2-3 at H = 128,
+    9-10 at H = 256, where layers 1 and 2 are restaged per chunk), with a ragged last chunk.
+    A walker's energy depends only on its own column of the MFMA tiles, so the energies of a
+    sample evaluated inside the large batch equal, bit for bit, those of the sample alone,
+    which match the oracle network."""
+    import pyvibdmc_amd as pv
+    m = pv.SurrogateModel(['H', 'H', 'O'], hidden=hidden, seed=7)
+    x = water_walkers(300_007, 8)
+    m.set_normalisation(x, P.morse_bend(x))  # the range covers every walker: no fallback
+    v_all = m.energy(x)
+    idx = np.concatenate([np.arange(0, len(x), 977), np.arange(len(x) - 40, len(x))])
+    xs = x[idx]
+    v_one = m.energy(xs)
+    np.testing.assert_array_equal(v_all[idx], v_one)
+    y, _ = oracle_net(m).forward(normalised_features(m, xs))
+    ref = y * m.desc.ystd + m.desc.ymean + O.quad_terms(normalised_features(m, xs)) @ np.array(m.desc.quad[:10])
+    err = np.linalg.norm(v_one - ref) / np.linalg.norm(ref - ref.mean())
+    assert err < 2e-2, err
