@@ -1,20 +1,7 @@
 // C ABI of the DMC propagation library: simulation handle, launch dispatch and the
 // auxiliary kernels (layout transposes, first-step vref, descendant weights, stand-alone
 // potential / trial evaluation).  Declarations and conventions: include/pyvibdmc_amd.h.
-#include <hip/hip_runtime.h>
-#include <hip/hip_ext.h>
-
-#include <algorithm>
-#include <cmath>
-#include <cstdio>
-#include <cstdlib>
-#include <cstring>
-#include <string>
-#include <type_traits>
-#include <vector>
-
-#include "../../include/pyvibdmc_amd.h"
-#include "pvd_kernels.hpp"
+#include "pvd_sim.hpp"
 
 using namespace pvd;
 
@@ -23,21 +10,15 @@ static_assert(sizeof(pvd_trial_desc) == sizeof(TrialParams), "trial layout");
 static_assert(PVD_MAX_COMP == MAXNP, "component bound");
 
 namespace {
-
 thread_local std::string g_err;
+}  // namespace
 
-enum : int { E_BAD = 7, E_HIP = 8 };
-
-int fail(int code, const std::string& msg) {
+int pvd::fail(int code, const std::string& msg) {
   g_err = msg;
   return code;
 }
 
-#define HIPCHK(x)                                                                         \
-  do {                                                                                    \
-    hipError_t e_ = (x);                                                                  \
-    if (e_ != hipSuccess) return fail(E_HIP, std::string(#x) + ": " + hipGetErrorString(e_)); \
-  } while (0)
+namespace {
 
 const char* status_text(int st) {
   switch (st) {
@@ -118,68 +99,6 @@ __global__ void k_desc_finish(const int* cnt, const unsigned long long* acc, dou
   }
 }
 
-// first-step energies and vref (pyvibdmc.py:760-769, 651-661)
-template <typename R, int NP, bool IS, bool EXT>
-__global__ __launch_bounds__(TILE) void k_start(StepArgs<R> A, int continuous) {
-  __shared__ double s_d[TILE / PVD_WAVE];
-  __shared__ unsigned s_last;
-  const int par = A.par;
-  Ctrl* C = A.ctrl;
-  const int n = C->n[par];
-  const int ntiles = (n + TILE - 1) / TILE;
-  const int tile = blockIdx.x;
-  if (tile >= ntiles) return;
-  const long long i = (long long)tile * TILE + threadIdx.x;
-  double E = 0.0, w = 0.0;
-  if (i < n) {
-    if constexpr (EXT) {
-      E = A.V_ext[i];
-    }
-    if constexpr (!EXT || IS) {
-      R x[NP];
-      load_walker<R, NP>(A, i, x);
-      if constexpr (!EXT) E = (double)pes_eval<R, NP>(A.pes, x);
-      if constexpr (IS) {
-        TrialOut<R, NP> t;
-        trial_eval<R, NP>(A.trial, x, t);
-        const R ke = local_kin<R, NP>(A, t.d2);
-        if constexpr (EXT) {
-          E = E + (double)ke;
-        } else {
-          E = (double)((R)E + ke);
-        }
-      }
-    }
-    w = continuous ? A.w[i] : 1.0;
-  }
-  const double swe = block_sum<TILE>(w * E, s_d);
-  const double sw = block_sum<TILE>(w, s_d);
-  if (threadIdx.x == 0) {
-    A.rec[tile].s0 = swe;
-    A.rec[tile].s1 = sw;
-    fence_release();
-    s_last = fetch_add_u(&C->done[par], 1u);
-  }
-  __syncthreads();
-  if (s_last != (unsigned)(ntiles - 1)) return;
-  fence_acquire();
-  double a0 = 0.0, a1 = 0.0;
-  for (int t = threadIdx.x; t < ntiles; t += TILE) {
-    a0 += A.rec[t].s0;
-    a1 += A.rec[t].s1;
-  }
-  a0 = block_sum<TILE>(a0, s_d);
-  a1 = block_sum<TILE>(a1, s_d);
-  if (threadIdx.x == 0) {
-    C->done[par] = 0u;
-    if (!continuous) {
-      C->vref[par] = a0 / (double)n - (A.alpha * ((double)(n - (int)A.n0) / A.n0));
-    } else {
-      C->vref[par] = a0 / a1 - (A.alpha * ((a1 - A.n0) / A.n0));
-    }
-  }
-}
-
 // stand-alone kernels on AoS walkers
 template <typename R, int NP>
 __global__ void k_pes_eval(PesParams P, const R* x, long long n, R* v) {
@@ -237,138 +156,6 @@ __global__ void k_trial_vjp(TrialParams T, const R* x, long long n, const R* go,
   for (int p = 0; p < NP; ++p) gx[i * NP + p] = s * o.d1[p];
 }
 
-unsigned nblk(long long n, int bs = 256) { return (unsigned)((n + bs - 1) / bs); }
-
-}  // namespace
-
-// ------------------------------------------------------------------ simulation state
-struct pvd_sim {
-  pvd_config cfg;
-  int np, M, d, prec, cont, is;
-  int mspec;  // ModelSpec of the fused step kernels (0 = runtime model parameters)
-  long long cap;
-  void* coords[2];
-  int* who[2];
-  bool who_on;
-  double* w;
-  double* E;
-  double* ke;
-  unsigned long long* kmask;
-  Ctrl* ctrl;
-  uint64_t* flags;
-  uint64_t* gflags;
-  TileRec* rec;
-  uint64_t* drec;
-  unsigned* accs;
-  double* tr_vref;
-  double* tr_pop;
-  double* tr_dteff;
-  int* tr_births;
-  int* tr_deaths;
-  long long tmax;
-  Piece* pend;
-  Piece* fin;
-  int* klist;
-  unsigned long long* killtag;
-  double* rstage;
-  Cand* cand;
-  int* asg;
-  int* cpy;
-  double* stage;
-  void* parent;
-  double* parent_w;
-  int n_parent;
-  double* desc;
-  int* cnt;
-  unsigned long long* dacc;
-  long long next_step;
-  unsigned epoch_next;
-  int status_seen;
-  int use_graphs;
-  int cus;                      // compute units of the device
-  const void* slot_fn[16];      // resident-workgroup counts of the discrete kernels used so far
-  int slot_val[16];
-  int n_slot;
-  hipGraphExec_t graph[2];  // replayed step graphs, without / with descendant tracking
-  // surrogate potential (external-potential simulations only)
-  int surr_on;
-  pvd_mlp surr;             // network description; its device parameters belong to the caller
-  pvd_pes_desc surr_fb;     // analytic fallback and judge potential
-  int judge_every, judge_stride;
-  double* vsur;             // surrogate V of the current walkers (cap)
-  double* jlog;             // judge records, 4 doubles each
-  int jcap, jn;
-};
-
-namespace {
-
-size_t esize(const pvd_sim* s) { return s->prec == PVD_FP64 ? 8 : 4; }
-
-// buffer holding the current walkers (discrete: by step parity; continuous: in place)
-int cur_buf(const pvd_sim* s) { return s->cont ? 0 : (int)(s->next_step & 1); }
-
-template <typename R>
-StepArgs<R> make_args(pvd_sim* s) {
-  StepArgs<R> A;
-  std::memset(&A, 0, sizeof(A));
-  const int par = (int)(s->next_step & 1);
-  const int cb = cur_buf(s);
-  A.cin = (const R*)s->coords[cb];
-  A.cout = (R*)(s->cont ? s->coords[0] : s->coords[cb ^ 1]);
-  A.cap = s->cap;
-  A.np = s->np;
-  if (s->who_on) {
-    A.who_in = s->who[cb];
-    A.who_out = s->cont ? s->who[0] : s->who[cb ^ 1];
-  }
-  A.w = s->w;
-  A.E = s->E;
-  A.ke = s->ke;
-  A.kmask = s->kmask;
-  A.ctrl = s->ctrl;
-  A.flags = s->flags;
-  A.gflags = s->gflags;
-  A.rec = s->rec;
-  A.drec = s->drec;
-  A.accs = s->accs;
-  A.tr_vref = s->tr_vref;
-  A.tr_pop = s->tr_pop;
-  A.tr_births = s->tr_births;
-  A.tr_deaths = s->tr_deaths;
-  A.tr_dteff = s->tr_dteff;
-  A.tmax = s->tmax;
-  A.pend = s->pend;
-  A.fin = s->fin;
-  A.klist = s->klist;
-  A.killtag = s->killtag;
-  A.rstage = s->rstage;
-  A.cand = s->cand;
-  A.asg = s->asg;
-  A.cpy = s->cpy;
-  std::memcpy(&A.pes, &s->cfg.pes, sizeof(PesParams));
-  std::memcpy(&A.trial, &s->cfg.trial, sizeof(TrialParams));
-  for (int p = 0; p < MAXNP; ++p) {
-    A.sigma[p] = s->cfg.sigma[p];
-    A.inv_m[p] = s->cfg.inv_mass[p];
-    A.h2s[p] = s->cfg.sigma[p] > 0.0 ? (float)(0.5 / (s->cfg.sigma[p] * s->cfg.sigma[p])) : 0.0f;
-  }
-  A.dt = s->cfg.delta_t;
-  A.alpha = s->cfg.alpha;
-  A.n0 = (double)s->cfg.num_walkers;
-  A.pop_lo = (double)s->cfg.num_walkers - (double)s->cfg.num_walkers * 0.5;
-  A.pop_hi = (double)s->cfg.num_walkers + (double)s->cfg.num_walkers * 0.5;
-  A.thresh_lo = s->cfg.thresh_lower;
-  A.rng.k0 = (uint32_t)s->cfg.seed;
-  A.rng.k1 = (uint32_t)(s->cfg.seed >> 32);
-  A.par = par;
-  const int be = s->cfg.branch_every > 0 ? s->cfg.branch_every : 1;
-  A.branch = (s->next_step % be) == 0 ? 1 : 0;
-  A.ms = s->M;
-  A.ds = s->d;
-  A.is_ = s->is;
-  return A;
-}
-
 // The compiled model specialisation for a configuration: MS_PES_WATER for the [H, H, O]
 // Morse+bend surface in 3 dimensions, plus MS_TRIAL_WATER when importance sampling uses the
 // matching two-bond / one-angle Gaussian trial; MS_PES_CLUSTER / MS_TRIAL_CLUSTER for the
@@ -389,378 +176,6 @@ int model_spec(const pvd_config& c, int np) {
   const bool tw = t.kind == TRIAL_GAUSS && t.nb == 2 && t.na == 1 && t.bi[0] == 0 && t.bj[0] == 2 &&
                   t.bi[1] == 1 && t.bj[1] == 2 && t.ai[0] == 0 && t.aj[0] == 2 && t.ak[0] == 1;
   return (pw ? MS_PES_WATER : MS_ANY) | (tw ? MS_TRIAL_WATER : MS_ANY);
-}
-
-// ---- launch helpers, instantiated over the component count ----------------------------
-#define PVD_NP_SWITCH(np, ...)                                  \
-  switch (np) {                                                 \
-    case 1: { constexpr int NP = 1; __VA_ARGS__; } break;       \
-    case 3: { constexpr int NP = 3; __VA_ARGS__; } break;       \
-    case 9: { constexpr int NP = 9; __VA_ARGS__; } break;       \
-    case 18: { constexpr int NP = 18; __VA_ARGS__; } break;     \
-    default: return fail(E_BAD, "fused kernels support 1, 3, 9 or 18 components per walker"); \
-  }
-
-// component count and model specialisation (the specialisations exist for 9 and 18
-// components)
-#define PVD_STEP_SWITCH(sim, ...)                                               \
-  PVD_NP_SWITCH((sim)->np, if constexpr (NP == 18) {                            \
-    if ((sim)->mspec == (MS_PES_CLUSTER | MS_TRIAL_CLUSTER)) {                  \
-      constexpr int MS = MS_PES_CLUSTER | MS_TRIAL_CLUSTER;                     \
-      __VA_ARGS__;                                                              \
-    } else if ((sim)->mspec == MS_TRIAL_CLUSTER) {                              \
-      constexpr int MS = MS_TRIAL_CLUSTER;                                      \
-      __VA_ARGS__;                                                              \
-    } else if ((sim)->mspec == MS_PES_CLUSTER) {                                \
-      constexpr int MS = MS_PES_CLUSTER;                                        \
-      __VA_ARGS__;                                                              \
-    } else {                                                                    \
-      constexpr int MS = MS_ANY;                                                \
-      __VA_ARGS__;                                                              \
-    }                                                                           \
-  } else if constexpr (NP == 9) {                                               \
-    if ((sim)->mspec == (MS_PES_WATER | MS_TRIAL_WATER)) {                      \
-      constexpr int MS = MS_PES_WATER | MS_TRIAL_WATER;                         \
-      __VA_ARGS__;                                                              \
-    } else if ((sim)->mspec == MS_TRIAL_WATER) {                                \
-      constexpr int MS = MS_TRIAL_WATER;                                        \
-      __VA_ARGS__;                                                              \
-    } else if ((sim)->mspec == MS_PES_WATER) {                                  \
-      constexpr int MS = MS_PES_WATER;                                          \
-      __VA_ARGS__;                                                              \
-    } else {                                                                    \
-      constexpr int MS = MS_ANY;                                                \
-      __VA_ARGS__;                                                              \
-    }                                                                           \
-  } else {                                                                      \
-    constexpr int MS = MS_ANY;                                                  \
-    __VA_ARGS__;                                                                \
-  })
-
-// The kernels of one fused step, in launch order, for the current parity.  Used both for
-// direct launches and as the node list of a replayed graph.
-struct Launch {
-  const void* fn;
-  dim3 grid, block;
-};
-
-// Resident workgroups of a discrete step kernel on this device (the tiling unit), cached.
-int disc_slots(pvd_sim* s, const void* fn) {
-  for (int i = 0; i < s->n_slot; ++i)
-    if (s->slot_fn[i] == fn) return s->slot_val[i];
-  int per_cu = 0;
-  if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, fn, TILE, 0) != hipSuccess || per_cu < 1) per_cu = 1;
-  const int v = per_cu * std::max(1, s->cus);
-  if (s->n_slot < 16) {
-    s->slot_fn[s->n_slot] = fn;
-    s->slot_val[s->n_slot] = v;
-    ++s->n_slot;
-  }
-  return v;
-}
-
-// Grid of a discrete step kernel: enough workgroups for the tiling of any population up to
-// the capacity (discrete_tiles); workgroups past the tile count exit at once.
-unsigned disc_grid(const pvd_sim* s, int slots, int W) {
-  const long long per_round = (long long)slots * TILE * W;
-  const long long rounds = (s->cap + per_round - 1) / per_round;
-  return (unsigned)std::max(1ll, rounds * slots);
-}
-
-template <typename R, bool INJ>
-int step_kernels(pvd_sim* s, StepArgs<R>& A, Launch* L, int& nl) {
-  const unsigned g = nblk(s->cap, TILE);
-  nl = 0;
-  const int pk = s->cfg.pes.kind;
-  if (pk == PES_EXTERNAL) return fail(E_BAD, "fused step needs a device potential");
-  auto add = [&](const void* fn, unsigned grid, unsigned block) { L[nl++] = Launch{fn, dim3(grid), dim3(block)}; };
-#define PVD_ADD(...) PVD_STEP_SWITCH(s, add((const void*)(__VA_ARGS__), g, TILE))
-  if (s->is) PVD_ADD(k_is_move<R, NP, INJ, MS>);
-  if (!s->cont) {
-    auto add_disc = [&](const void* fn, int W) {
-      A.slots = disc_slots(s, fn);
-      add(fn, disc_grid(s, A.slots, W), TILE);
-    };
-    if (s->is) {
-      PVD_STEP_SWITCH(s, add_disc((const void*)k_discrete<R, NP, INJ, true, false, MS>, disc_wpt<R, NP, false, true>()));
-    } else {
-      PVD_STEP_SWITCH(s, add_disc((const void*)k_discrete<R, NP, INJ, false, false, (MS & MS_PES_ANY_SPEC)>,
-                                  disc_wpt<R, NP, false>()));
-    }
-  } else {
-    if (s->is) {
-      PVD_ADD(k_cont<R, NP, INJ, true, false, MS>);
-    } else {
-      PVD_ADD(k_cont<R, NP, INJ, false, false, (MS & MS_PES_ANY_SPEC)>);
-    }
-    add((const void*)k_cont_replay<R>, 1, RP_NT);
-    add((const void*)k_cont_copy<R>, KILL_CAP / TILE, TILE);
-  }
-#undef PVD_ADD
-  return 0;
-}
-
-template <typename R, bool INJ>
-int launch_fused_step(pvd_sim* s, StepArgs<R>& A, hipStream_t st) {
-  Launch L[8];
-  int nl = 0;
-  int rc = step_kernels<R, INJ>(s, A, L, nl);
-  if (rc) return rc;
-  void* args[] = {&A};
-  for (int i = 0; i < nl; ++i) HIPCHK(hipLaunchKernel(L[i].fn, L[i].grid, L[i].block, args, 0, st));
-  return 0;
-}
-
-// A graph of PVD_GRAPH_STEPS consecutive steps starting at an even step: the kernels are
-// parameter-identical from one even/odd pair to the next (all step-dependent state is read
-// from the device control block), so one instantiated graph replays any number of times.
-constexpr int PVD_GRAPH_STEPS = 64;
-
-template <typename R>
-int build_graph(pvd_sim* s, hipGraphExec_t* out) {
-  hipGraph_t g;
-  HIPCHK(hipGraphCreate(&g, 0));
-  const long long saved = s->next_step;
-  const long long base = saved & ~1ll;
-  hipGraphNode_t prev = nullptr;
-  int rc = 0;
-  for (int k = 0; k < PVD_GRAPH_STEPS && rc == 0; ++k) {
-    s->next_step = base + k;
-    StepArgs<R> A = make_args<R>(s);
-    Launch L[8];
-    int nl = 0;
-    rc = step_kernels<R, false>(s, A, L, nl);
-    for (int i = 0; i < nl && rc == 0; ++i) {
-      hipKernelNodeParams p;
-      std::memset(&p, 0, sizeof(p));
-      void* args[] = {&A};
-      p.func = const_cast<void*>(L[i].fn);
-      p.gridDim = L[i].grid;
-      p.blockDim = L[i].block;
-      p.sharedMemBytes = 0;
-      p.kernelParams = args;
-      p.extra = nullptr;
-      hipGraphNode_t node;
-      hipError_t e = hipGraphAddKernelNode(&node, g, prev ? &prev : nullptr, prev ? 1 : 0, &p);
-      if (e != hipSuccess) rc = fail(E_HIP, std::string("hipGraphAddKernelNode: ") + hipGetErrorString(e));
-      prev = node;
-    }
-  }
-  s->next_step = saved;
-  if (rc == 0) {
-    hipError_t e = hipGraphInstantiate(out, g, nullptr, nullptr, 0);
-    if (e != hipSuccess) rc = fail(E_HIP, std::string("hipGraphInstantiate: ") + hipGetErrorString(e));
-  }
-  (void)hipGraphDestroy(g);
-  return rc;
-}
-
-template <typename R>
-int do_move(pvd_sim* s, const double* disp, const double* um, hipStream_t st);
-template <typename R>
-int do_branch_ext(pvd_sim* s, const double* v, const double* ub, hipStream_t st);
-
-// One step with the surrogate potential: move in place, network energy (+ analytic fallback)
-// on the device planes, judge on judge steps, branch.
-template <typename R>
-int surrogate_step(pvd_sim* s, hipStream_t st) {
-  int rc = do_move<R>(s, nullptr, nullptr, st);
-  if (rc) return rc;
-  const int cb = cur_buf(s);
-  const int32_t* n_dev = &s->ctrl->n[(int)(s->next_step & 1)];
-  const int prec = s->prec == PVD_FP64 ? PVD_FP64 : PVD_FP32;
-  rc = pvd_mlp_energy(&s->surr, prec, s->M, s->coords[cb], s->cap, n_dev, s->cap, s->vsur, &s->surr_fb, st);
-  if (rc) return fail(E_BAD, std::string("surrogate energy: ") + pvd_mlp_last_error());
-  if (s->judge_every > 0 && (s->next_step + 1) % s->judge_every == 0 && s->jn < s->jcap) {
-    rc = pvd_mlp_judge(&s->surr_fb, prec, s->M, s->coords[cb], s->cap, n_dev, s->cap, s->vsur, s->judge_stride,
-                       s->next_step, s->jlog + 4ll * s->jn, st);
-    if (rc) return fail(E_BAD, std::string("surrogate judge: ") + pvd_mlp_last_error());
-    ++s->jn;
-  }
-  return do_branch_ext<R>(s, s->vsur, nullptr, st);
-}
-
-template <typename R>
-int run_steps(pvd_sim* s, int n_steps, const double* disp, const double* um, const double* ub,
-              hipStream_t st) {
-  if (s->surr_on) {
-    if (disp) return fail(E_BAD, "injected draws with a surrogate potential: use move / branch_external");
-    for (int k = 0; k < n_steps; ++k) {
-      const int rc = surrogate_step<R>(s, st);
-      if (rc) return rc;
-    }
-    return 0;
-  }
-  int k = 0;
-  const bool graphs = !disp && s->use_graphs && (s->cfg.branch_every <= 1) && n_steps >= PVD_GRAPH_STEPS + 1;
-  if (graphs) {
-    if (s->next_step & 1) {
-      StepArgs<R> A = make_args<R>(s);
-      int rc = launch_fused_step<R, false>(s, A, st);
-      if (rc) return rc;
-      s->next_step += 1;
-      s->epoch_next += 1;
-      ++k;
-    }
-    const int slot = s->who_on ? 1 : 0;
-    if (!s->graph[slot]) {
-      int rc = build_graph<R>(s, &s->graph[slot]);
-      if (rc) return rc;
-    }
-    for (; n_steps - k >= PVD_GRAPH_STEPS; k += PVD_GRAPH_STEPS) {
-      HIPCHK(hipGraphLaunch(s->graph[slot], st));
-      s->next_step += PVD_GRAPH_STEPS;
-      s->epoch_next += PVD_GRAPH_STEPS;
-    }
-  }
-  for (; k < n_steps; ++k) {
-    StepArgs<R> A = make_args<R>(s);
-    int rc;
-    if (disp) {
-      A.inj_disp = disp;
-      A.inj_umet = um;
-      A.inj_ubr = ub;
-      rc = launch_fused_step<R, true>(s, A, st);
-    } else {
-      rc = launch_fused_step<R, false>(s, A, st);
-    }
-    if (rc) return rc;
-    s->next_step += 1;
-    s->epoch_next += 1;
-  }
-  return 0;
-}
-
-// Eager steps with an event pair on every kernel dispatch (mean duration per kernel slot).
-template <typename R>
-int run_steps_timed(pvd_sim* s, int n_steps, hipStream_t st, float* avg_ms, int32_t* n_kernels) {
-  Launch L[8];
-  int nl = 0;
-  StepArgs<R> A0 = make_args<R>(s);
-  int rc = step_kernels<R, false>(s, A0, L, nl);
-  if (rc) return rc;
-  std::vector<hipEvent_t> ev((size_t)2 * nl * n_steps, nullptr);
-  auto release = [&]() {
-    for (hipEvent_t e : ev)
-      if (e) (void)hipEventDestroy(e);
-  };
-  for (hipEvent_t& e : ev) {
-    if (hipEventCreate(&e) != hipSuccess) {
-      release();
-      return fail(E_HIP, "hipEventCreate failed");
-    }
-  }
-  for (int k = 0; k < n_steps; ++k) {
-    StepArgs<R> A = make_args<R>(s);
-    A.slots = A0.slots;
-    void* args[] = {&A};
-    for (int i = 0; i < nl; ++i) {
-      // start / stop events recorded by the dispatch itself (kernel execution only)
-      hipEvent_t a = ev[(size_t)2 * (k * nl + i)], b = ev[(size_t)2 * (k * nl + i) + 1];
-      hipError_t e = hipExtLaunchKernel(L[i].fn, L[i].grid, L[i].block, args, 0, st, a, b, 0);
-      if (e != hipSuccess) {
-        release();
-        return fail(E_HIP, std::string("timed launch: ") + hipGetErrorString(e));
-      }
-    }
-    s->next_step += 1;
-    s->epoch_next += 1;
-  }
-  hipError_t e = hipStreamSynchronize(st);
-  for (int i = 0; i < nl && e == hipSuccess; ++i) {
-    double sum = 0.0;
-    for (int k = 0; k < n_steps && e == hipSuccess; ++k) {
-      float ms = 0.f;
-      e = hipEventElapsedTime(&ms, ev[(size_t)2 * (k * nl + i)], ev[(size_t)2 * (k * nl + i) + 1]);
-      sum += ms;
-    }
-    avg_ms[i] = n_steps > 0 ? (float)(sum / n_steps) : 0.f;
-  }
-  release();
-  if (e != hipSuccess) return fail(E_HIP, std::string("timed launch events: ") + hipGetErrorString(e));
-  *n_kernels = nl;
-  return 0;
-}
-
-template <typename R>
-int do_move(pvd_sim* s, const double* disp, const double* um, hipStream_t st) {
-  StepArgs<R> A = make_args<R>(s);
-  // the move happens in place in the current buffer
-  A.cout = (R*)A.cin;
-  A.inj_disp = disp;
-  A.inj_umet = um;
-  const unsigned g = nblk(s->cap, TILE);
-  if (s->is) {
-    if (disp) {
-      PVD_STEP_SWITCH(s, hipLaunchKernelGGL((k_move<R, NP, true, true, (MS & MS_TRIAL_ANY_SPEC)>), dim3(g), dim3(TILE), 0, st, A));
-    } else {
-      PVD_STEP_SWITCH(s, hipLaunchKernelGGL((k_move<R, NP, false, true, (MS & MS_TRIAL_ANY_SPEC)>), dim3(g), dim3(TILE), 0, st, A));
-    }
-  } else {
-    if (disp) hipLaunchKernelGGL((k_move<R, 1, true, false>), dim3(g), dim3(TILE), 0, st, A);
-    else hipLaunchKernelGGL((k_move<R, 1, false, false>), dim3(g), dim3(TILE), 0, st, A);
-  }
-  HIPCHK(hipGetLastError());
-  return 0;
-}
-
-template <typename R>
-int do_branch_ext(pvd_sim* s, const double* v, const double* ub, hipStream_t st) {
-  StepArgs<R> A = make_args<R>(s);
-  A.V_ext = v;
-  A.inj_ubr = ub;
-  unsigned g = nblk(s->cap, TILE);
-  if (!s->cont) {
-    // compile-time component count where one is instantiated, else the run-time form (NP = 0)
-    const void* fn = nullptr;
-    auto pick = [&](auto np_tag) {
-      constexpr int NP = decltype(np_tag)::value;
-      fn = ub ? (s->is ? (const void*)k_discrete<R, NP, true, true, true> : (const void*)k_discrete<R, NP, true, false, true>)
-              : (s->is ? (const void*)k_discrete<R, NP, false, true, true> : (const void*)k_discrete<R, NP, false, false, true>);
-    };
-    switch (s->np) {
-      case 1: pick(std::integral_constant<int, 1>()); break;
-      case 3: pick(std::integral_constant<int, 3>()); break;
-      case 9: pick(std::integral_constant<int, 9>()); break;
-      case 18: pick(std::integral_constant<int, 18>()); break;
-      default: pick(std::integral_constant<int, 0>()); break;
-    }
-    A.slots = disc_slots(s, fn);
-    g = disc_grid(s, A.slots, disc_wpt<R, 1, true>());
-    void* args[] = {&A};
-    HIPCHK(hipLaunchKernel(fn, dim3(g), dim3(TILE), args, 0, st));
-  } else {
-    if (s->is) hipLaunchKernelGGL((k_cont<R, 1, false, true, true>), dim3(g), dim3(TILE), 0, st, A);
-    else hipLaunchKernelGGL((k_cont<R, 1, false, false, true>), dim3(g), dim3(TILE), 0, st, A);
-    hipLaunchKernelGGL((k_cont_replay<R>), dim3(1), dim3(RP_NT), 0, st, A);
-    hipLaunchKernelGGL((k_cont_copy<R>), dim3(KILL_CAP / TILE), dim3(TILE), 0, st, A);
-  }
-  HIPCHK(hipGetLastError());
-  s->next_step += 1;
-  s->epoch_next += 1;
-  return 0;
-}
-
-template <typename R>
-int do_start(pvd_sim* s, const double* v, hipStream_t st) {
-  StepArgs<R> A = make_args<R>(s);
-  A.V_ext = v;
-  const unsigned g = nblk(s->cap, TILE);
-  if (v) {
-    if (s->is) {
-      PVD_NP_SWITCH(s->np, hipLaunchKernelGGL((k_start<R, NP, true, true>), dim3(g), dim3(TILE), 0, st, A, s->cont));
-    } else {
-      hipLaunchKernelGGL((k_start<R, 1, false, true>), dim3(g), dim3(TILE), 0, st, A, s->cont);
-    }
-  } else {
-    if (s->is) {
-      PVD_NP_SWITCH(s->np, hipLaunchKernelGGL((k_start<R, NP, true, false>), dim3(g), dim3(TILE), 0, st, A, s->cont));
-    } else {
-      PVD_NP_SWITCH(s->np, hipLaunchKernelGGL((k_start<R, NP, false, false>), dim3(g), dim3(TILE), 0, st, A, s->cont));
-    }
-  }
-  HIPCHK(hipGetLastError());
-  return 0;
 }
 
 int read_ctrl(pvd_sim* s, Ctrl& c, hipStream_t st) {
