@@ -1,6 +1,8 @@
 // C ABI of the DMC propagation library: simulation handle, launch dispatch and the
 // auxiliary kernels (layout transposes, first-step vref, descendant weights, stand-alone
 // potential / trial evaluation).  Declarations and conventions: include/pyvibdmc_amd.h.
+#include <initializer_list>
+
 #include "pvd_sim.hpp"
 
 using namespace pvd;
@@ -160,7 +162,18 @@ __global__ void k_trial_vjp(TrialParams T, const R* x, long long n, const R* go,
 // Morse+bend surface in 3 dimensions, plus MS_TRIAL_WATER when importance sampling uses the
 // matching two-bond / one-angle Gaussian trial; MS_PES_CLUSTER / MS_TRIAL_CLUSTER for the
 // [O, H, H, O, H, H] cluster surface and its per-monomer trial.  Other layouts keep the
-// runtime parameters.
+// runtime parameters.  The single-precision trial specialisations weight the Laplacian and
+// the drift with one 1/m for every hydrogen and one for every oxygen, so they are chosen
+// only when the masses of those atoms are equal (an isotopologue such as HOD, or a mixed
+// H2O / D2O cluster, takes the per-atom path).
+bool same_mass(const pvd_config& c, std::initializer_list<int> atoms) {
+  const double m0 = c.inv_mass[3 * *atoms.begin()];
+  for (int a : atoms)
+    for (int k = 0; k < 3; ++k)
+      if (c.inv_mass[3 * a + k] != m0) return false;
+  return true;
+}
+
 int model_spec(const pvd_config& c, int np) {
   const pvd_pes_desc& p = c.pes;
   const pvd_trial_desc& t = c.trial;
@@ -168,13 +181,15 @@ int model_spec(const pvd_config& c, int np) {
     const bool pc = p.kind == PES_CLUSTER6;
     const bool tc = t.kind == TRIAL_GAUSS && t.nb == 4 && t.na == 2 && t.bi[0] == 1 && t.bj[0] == 0 &&
                     t.bi[1] == 2 && t.bj[1] == 0 && t.bi[2] == 4 && t.bj[2] == 3 && t.bi[3] == 5 && t.bj[3] == 3 &&
-                    t.ai[0] == 1 && t.aj[0] == 0 && t.ak[0] == 2 && t.ai[1] == 4 && t.aj[1] == 3 && t.ak[1] == 5;
+                    t.ai[0] == 1 && t.aj[0] == 0 && t.ak[0] == 2 && t.ai[1] == 4 && t.aj[1] == 3 && t.ak[1] == 5 &&
+                    same_mass(c, {1, 2, 4, 5}) && same_mass(c, {0, 3});
     return (pc ? MS_PES_CLUSTER : MS_ANY) | (tc ? MS_TRIAL_CLUSTER : MS_ANY);
   }
   if (np != 9 || c.n_atoms != 3 || c.n_dim != 3) return MS_ANY;
   const bool pw = p.kind == PES_MORSE_BEND && p.idx[0] == 0 && p.idx[1] == 1 && p.idx[2] == 2;
   const bool tw = t.kind == TRIAL_GAUSS && t.nb == 2 && t.na == 1 && t.bi[0] == 0 && t.bj[0] == 2 &&
-                  t.bi[1] == 1 && t.bj[1] == 2 && t.ai[0] == 0 && t.aj[0] == 2 && t.ak[0] == 1;
+                  t.bi[1] == 1 && t.bj[1] == 2 && t.ai[0] == 0 && t.aj[0] == 2 && t.ak[0] == 1 &&
+                  same_mass(c, {0, 1}) && same_mass(c, {2});
   return (pw ? MS_PES_WATER : MS_ANY) | (tw ? MS_TRIAL_WATER : MS_ANY);
 }
 

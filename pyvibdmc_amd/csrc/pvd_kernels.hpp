@@ -612,7 +612,10 @@ PVD_DEV void finalize_discrete(const StepArgs<R>& A, Ctrl* C, int par, int n, in
     A.tr_births[step] = nb;
     A.tr_deaths[step] = nd;
     if (IS) {
-      C->eff_t = (step == 0 ? 0.0 : C->eff_t) + dte;
+      // the effective time advances on branching steps only (update_effetive_timestep is
+      // called from birth_or_death, pyvibdmc.py:384-385, 372-378); a step without branching
+      // leaves its entry at zero, and the next branching step adds to that zero
+      C->eff_t = A.branch ? (step == 0 ? 0.0 : C->eff_t) + dte : 0.0;
       A.tr_dteff[step] = C->eff_t;
     }
   }
@@ -937,11 +940,8 @@ __global__ __launch_bounds__(TILE) void k_cont(StepArgs<R> A) {
       E = A.V_ext[i];
       if (IS) E = E + A.ke[i];
     } else if constexpr (IS) {
-      // moved, and its local energy computed, by k_is_move
-      R x[NP];
-      load_walker<R, NP>(A, i, x);
-#pragma unroll
-      for (int p = 0; p < NP; ++p) A.cout[p * A.cap + i] = x[p];
+      // moved in place, and its local energy computed, by k_is_move (continuous weighting
+      // keeps one walker buffer, so the coordinates need no copy)
       E = A.E[i];
     } else {
       R x[NP], disp[NP];
@@ -1233,7 +1233,10 @@ PVD_DEV int replay_candidates(const StepArgs<R>& A, int K, int nlists, const int
       base = __shfl(base, lead, PVD_WAVE);
       const int o = base + __popcll(bal & ((1ull << lane_id()) - 1ull));
       if (in && o < CAND_CAP) {
-        key[o] = MANT - ((unsigned long long)__double_as_longlong(w) & MANT);
+        // decreasing mantissa, then decreasing exponent: in a run of equal mantissas the
+        // adjacent pair of largest minimum exponent is the run's first pair
+        const unsigned long long wb = (unsigned long long)__double_as_longlong(w);
+        key[o] = ((MANT - (wb & MANT)) << 11) | (0x7FFull - ((wb >> 52) & 0x7FFull));
         ord[o] = (unsigned short)pq;  // < 2 * LIST_CAP = 65536
       }
     }
@@ -1242,11 +1245,11 @@ PVD_DEV int replay_candidates(const StepArgs<R>& A, int K, int nlists, const int
   const int C = s_sh[3];
   if (C > CAND_CAP) return RC_MANY;
   if (C == 0) return RC_FEW;
-  // 2. candidates by decreasing mantissa: counting sort on the top 12 bits of the key
-  //    (buckets in LDS), then each entry ranked inside its bucket; a bitonic sort when one
-  //    bucket holds too many
+  // 2. candidates by decreasing mantissa (then exponent): counting sort on the top 12 bits
+  //    of the 63-bit key (buckets in LDS), then each entry ranked inside its bucket; a
+  //    bitonic sort when one bucket holds too many
   constexpr int KPT = CAND_CAP / NT;
-  constexpr int BSH = 52 - 12;
+  constexpr int BSH = 63 - 12;
   for (int b = tid; b < RC_BUCKETS; b += NT) bcnt[b] = 0;
   __syncthreads();
   int bpos[KPT];
@@ -1348,7 +1351,9 @@ PVD_DEV int replay_candidates(const StepArgs<R>& A, int K, int nlists, const int
       ej[s] = dexp(c.w);
       mj[s] = (unsigned long long)__double_as_longlong(c.w) & MANT;
       etop = max(etop, ej[s]);
-      if (j + 1 < C && key[j + 1] == key[j]) tie_e = max(tie_e, min(ej[s], dexp(A.cand[ord[j + 1]].w)));
+      // equal mantissas meet in every octave at or below the smaller exponent
+      if (j + 1 < C && (key[j + 1] >> 11) == (key[j] >> 11))
+        tie_e = max(tie_e, min(ej[s], dexp(A.cand[ord[j + 1]].w)));
     }
   }
   etop = block_max<NT>(etop, s_i);

@@ -214,3 +214,22 @@ def test_is_fp32_cluster_matches_fp64_zpe():
         sim.propagate()
         z[prec] = np.mean(sim._vref_vs_tau[500:]) / WN
     assert abs(z['fp32'] - z['fp64']) < 4.0, z
+
+
+@pytest.mark.parametrize('atoms', [['H', 'D', 'O'], ['D', 'H', 'O']])
+def test_is_fp32_isotopologue_tracks_fp64(atoms):
+    """HOD / DHO with the [H, H, O] Gaussian trial: the single-precision move weights the
+    drift and the Laplacian with each atom's own 1/m.  On the same injected draws the FP32
+    run follows the FP64 run (vref to 2e-5 hartree for 30 steps); weighting the deuteron
+    with the proton's 1/m moves the local energy by ~1e-3 hartree."""
+    import pyvibdmc_amd as pv
+    from oracle.dmc import LegacyDraws
+    vref = {}
+    for prec in ('fp32', 'fp64'):
+        sim = pv.DMC_Sim(num_walkers=2000, num_timesteps=30, atoms=atoms, delta_t=5,
+                         potential=pv.MorseBendTriatomic(), start_structures=water(2000, 17, 0.05),
+                         imp_samp=pv.water_trial(), precision=prec, rng=LegacyDraws(23), write_files=False,
+                         equil_steps=10 ** 9, wfn_every=10 ** 9, chkpt_every=10 ** 9)
+        sim.propagate()
+        vref[prec] = sim._vref_vs_tau.copy()
+    np.testing.assert_allclose(vref['fp32'], vref['fp64'], rtol=0, atol=2e-5)

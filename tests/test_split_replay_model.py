@@ -60,8 +60,9 @@ def candidate_replay(w, thresh, T0):
     cand = [i for i in range(n) if w[i] >= T0]
     if K == 0 or not cand:
         return None
-    # families by decreasing mantissa (ties in any order: they make the model decline)
-    cs = sorted(cand, key=lambda i: -(_bits(w[i]) & MANT))
+    # families by decreasing mantissa, then decreasing exponent (the kernel's 63-bit key);
+    # equal weights in any order: they make the model decline
+    cs = sorted(cand, key=lambda i: (-(_bits(w[i]) & MANT), -_exp(w[i])))
     ms = [_bits(w[o]) & MANT for o in cs]
     es = [_exp(w[o]) for o in cs]
     mT, eT = _bits(T0) & MANT, _exp(T0)
@@ -140,3 +141,17 @@ def test_candidate_replay_reproduces_the_sequential_loop(seed):
         np.testing.assert_array_equal(got[1], ref_src)
     # the model must serve most cases (it declines only on ties and on too high a T0)
     assert served > 300, (served, declined)
+
+
+def test_equal_mantissas_at_three_exponents_decline():
+    """Families of equal mantissa at exponents e+1, e, e+1 (weights 3, 1.5, 3): the tie that
+    matters is between the two weight-3 families, which meet in the octave the single kill
+    uses.  Sorted by exponent inside the run of equal mantissas they are adjacent, so the
+    check sees the tie and the replay declines (the loop copies the first weight-3 slot).
+    In the order e+1, e, e+1 the adjacent pairs only show exponent e and the tie is missed."""
+    w = np.array([3.0, 1.5, 3.0, 0.001, 0.8, 0.7])
+    assert candidate_replay(w, 0.01, 1.0) is None
+    ref_w, ref_src = oracle_split(w, 0.01)
+    assert ref_src[3] == 0 and ref_w[0] == ref_w[3] == 1.5
+    es = [1, 0, 1]
+    assert max(min(es[j], es[j + 1]) for j in range(2)) < 1
