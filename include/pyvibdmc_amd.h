@@ -43,7 +43,12 @@ enum pvd_pes_kind {
   PVD_PES_MORSE_BEND = 3, /* idx = (h1, h2, o); p = (De, a, re, kb, theta0)             */
   PVD_PES_CLUSTER6 = 4    /* [O,H,H,O,H,H]; p = (De,a,re,kb,theta0, De_oo,a_oo,re_oo, qO,qH) */
 };
-enum pvd_trial_kind { PVD_TRIAL_NONE = 0, PVD_TRIAL_HARMONIC_1D = 1, PVD_TRIAL_GAUSS = 2 };
+enum pvd_trial_kind {
+  PVD_TRIAL_NONE = 0,
+  PVD_TRIAL_HARMONIC_1D = 1,
+  PVD_TRIAL_GAUSS = 2,
+  PVD_TRIAL_HOST = 3 /* psi_T evaluated by the caller: pvd_sim_is_propose / pvd_sim_is_accept */
+};
 
 /* Device potential.  Replaces the user function behind Potential*.getpot
  * (potential_manager.py:71-99, 144-174, 237-253). */
@@ -130,6 +135,30 @@ int pvd_sim_step_injected(pvd_sim* sim, const double* disp, const double* u_metr
 int pvd_sim_move(pvd_sim* sim, const double* disp, const double* u_metro, void* stream);
 int pvd_sim_branch_external(pvd_sim* sim, const double* v_dev, const double* u_branch,
                             void* stream);
+/* Importance sampling with a trial wave function evaluated by the caller (trial.kind =
+ * PVD_TRIAL_HOST): imp_move_randomly (pyvibdmc.py:549-612) split around its plugin calls
+ * self.impsamp.drift(cds) -> ImpSampManager.call_trial / call_derivs (imp_samp.py:21-26,
+ * imp_samp_manager.py:92-139, 197-224).  The library keeps the move's caches (psi_1, f_x,
+ * psi_sec_der) per walker and carries them through branching (pyvibdmc.py:424-427, 353-356).
+ * All arrays are device float64 rows of the current population n: (n, M, d) or (n).
+ *  - pvd_sim_is_cache: caches from the caller's drift(x) of the current walkers; x (or NULL)
+ *    replaces the walkers (a finite-difference call perturbs its input in place); stores the
+ *    local kinetic energy of d2psi for the first-step energies (pyvibdmc.py:760-769).
+ *  - pvd_sim_is_propose: y = x + disp + (1/m) f_x dt (pyvibdmc.py:556-564); disp as in
+ *    pvd_sim_step_injected, or NULL for Philox draws.
+ *  - pvd_sim_is_accept: Metropolis of imp_samp.py:28-47 against the caller's psi / dpsi /
+ *    d2psi at y (y as the caller left it), acceptance, caches and local kinetic energy
+ *    (pyvibdmc.py:566-609, 807-809); u_metro device or NULL (Philox).  Then a potential and
+ *    pvd_sim_branch_external complete the step. */
+int pvd_sim_is_cache(pvd_sim* sim, const double* x, const double* psi, const double* dpsi,
+                     const double* d2psi, void* stream);
+int pvd_sim_is_propose(pvd_sim* sim, const double* disp, double* y, void* stream);
+int pvd_sim_is_accept(pvd_sim* sim, const double* y, const double* psi_y, const double* dpsi_y,
+                      const double* d2psi_y, const double* u_metro, void* stream);
+/* The simulation's device potential at its current walkers (potential.getpot(
+ * self._walker_coords), pyvibdmc.py:786-793): v (device, float64, n). */
+int pvd_sim_pes(pvd_sim* sim, double* v, void* stream);
+
 /* Current walkers as SoA planes: component p of walker i at planes[p*stride + i];
  * element type float64 (PVD_FP64) or float32 (PVD_FP32). */
 int pvd_sim_planes(pvd_sim* sim, void** planes, int64_t* stride, int32_t* n, void* stream);

@@ -15,7 +15,7 @@ MAX_COMP = 96
 PVD_FP64, PVD_FP32 = 0, 1
 PVD_DISCRETE, PVD_CONTINUOUS = 0, 1
 PES_EXTERNAL, PES_HARMONIC_X, PES_MORSE_1D, PES_MORSE_BEND, PES_CLUSTER6 = 0, 1, 2, 3, 4
-TRIAL_NONE, TRIAL_HARMONIC_1D, TRIAL_GAUSS = 0, 1, 2
+TRIAL_NONE, TRIAL_HARMONIC_1D, TRIAL_GAUSS, TRIAL_HOST = 0, 1, 2, 3
 
 ST_POP, ST_WEIGHT, ST_CAPACITY, ST_BRANCH, ST_SPIN, E_BAD, E_HIP = 1, 3, 4, 5, 6, 7, 8
 MASSIVE = "Massive walker birth or death event!!!!!!! Dying..."
@@ -69,6 +69,10 @@ _SIGS = {
     'pvd_sim_step_injected': (C.c_int, [_P, _P, _P, _P, _P]),
     'pvd_sim_move': (C.c_int, [_P, _P, _P, _P]),
     'pvd_sim_branch_external': (C.c_int, [_P, _P, _P, _P]),
+    'pvd_sim_is_cache': (C.c_int, [_P, _P, _P, _P, _P, _P]),
+    'pvd_sim_is_propose': (C.c_int, [_P, _P, _P, _P]),
+    'pvd_sim_is_accept': (C.c_int, [_P, _P, _P, _P, _P, _P, _P]),
+    'pvd_sim_pes': (C.c_int, [_P, _P, _P]),
     'pvd_sim_planes': (C.c_int, [_P, C.POINTER(_P), C.POINTER(C.c_int64), C.POINTER(C.c_int32), _P]),
     'pvd_sim_status': (C.c_int, [_P, C.POINTER(C.c_int32), C.POINTER(C.c_int64), C.POINTER(C.c_int32),
                                  C.POINTER(C.c_double), C.POINTER(C.c_int64), _P]),

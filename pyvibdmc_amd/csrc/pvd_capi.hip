@@ -238,7 +238,7 @@ void free_sim(pvd_sim* s) {
   void* ptrs[] = {s->coords[0], s->coords[1], s->who[0], s->who[1], s->w, s->E, s->ke, s->kmask,
                   s->ctrl, s->flags, s->gflags, s->rec, s->drec, s->accs, s->tr_vref, s->tr_pop, s->tr_dteff, s->tr_births,
                   s->tr_deaths, s->pend, s->fin, s->klist, s->killtag, s->rstage, s->cand, s->asg, s->cpy, s->stage, s->parent, s->parent_w, s->desc,
-                  s->cnt, s->dacc, s->vsur, s->jlog};
+                  s->cnt, s->dacc, s->vsur, s->jlog, s->xc[0], s->xc[1]};
   for (void* p : ptrs)
     if (p) (void)hipFree(p);
   delete s;
@@ -262,7 +262,7 @@ int pvd_sim_create(const pvd_config* cfg, pvd_sim** out) {
   if (cfg->precision != PVD_FP64 && cfg->precision != PVD_FP32) return fail(E_BAD, "bad precision");
   if (cfg->weighting != PVD_DISCRETE && cfg->weighting != PVD_CONTINUOUS)
     return fail(E_BAD, "bad weighting");
-  if (cfg->trial.kind != TRIAL_NONE && np > PVD_FUSED_MAX_COMP)
+  if (cfg->trial.kind != TRIAL_NONE && cfg->trial.kind != TRIAL_HOST && np > PVD_FUSED_MAX_COMP)
     return fail(E_BAD, "importance sampling supports at most 18 components per walker");
   pvd_sim* s = new pvd_sim();
   std::memset((void*)s, 0, sizeof(pvd_sim));
@@ -273,6 +273,8 @@ int pvd_sim_create(const pvd_config* cfg, pvd_sim** out) {
   s->prec = cfg->precision;
   s->cont = cfg->weighting == PVD_CONTINUOUS;
   s->is = cfg->trial.kind != TRIAL_NONE;
+  s->htrial = cfg->trial.kind == TRIAL_HOST;
+  s->nx = s->htrial ? 2 * np + 1 : 0;
   s->mspec = model_spec(*cfg, np);
   long long cap = cfg->capacity;
   const long long need = (long long)std::floor(1.5 * cfg->num_walkers) + 1;
@@ -335,9 +337,13 @@ int pvd_sim_create(const pvd_config* cfg, pvd_sim** out) {
     ALLOC(s->asg, sizeof(int) * KILL_CAP);
     ALLOC(s->cpy, sizeof(int) * 2 * KILL_CAP);
     ALLOC(s->killtag, sizeof(unsigned long long) * cap);
-    ALLOC(s->rstage, sizeof(double) * (size_t)RP_FCAP * (np + 2));
+    ALLOC(s->rstage, sizeof(double) * (size_t)RP_FCAP * (np + 2 + (cfg->trial.kind == TRIAL_HOST ? 2 * np + 1 : 0)));
   }
   ALLOC(s->stage, sizeof(double) * np * cap);
+  if (s->htrial) {
+    ALLOC(s->xc[0], sizeof(double) * s->nx * cap);
+    if (!s->cont) ALLOC(s->xc[1], sizeof(double) * s->nx * cap);
+  }
 #undef ALLOC
   hipError_t e = hipMemset(s->flags, 0, sizeof(uint64_t) * ntiles);
   if (e == hipSuccess) e = hipMemset(s->gflags, 0, sizeof(uint64_t) * (ntiles / LB_GROUP + 1));
@@ -493,6 +499,31 @@ int pvd_sim_branch_external(pvd_sim* s, const double* v, const double* ub, void*
   if (!s || !v) return fail(E_BAD, "null argument");
   return s->prec == PVD_FP64 ? do_branch_ext<double>(s, v, ub, (hipStream_t)stream)
                              : do_branch_ext<float>(s, v, ub, (hipStream_t)stream);
+}
+
+int pvd_sim_is_cache(pvd_sim* s, const double* x, const double* psi, const double* dpsi, const double* d2psi,
+                     void* stream) {
+  if (!s || !psi || !dpsi || !d2psi) return fail(E_BAD, "null argument");
+  return s->prec == PVD_FP64 ? do_is_cache<double>(s, x, psi, dpsi, d2psi, (hipStream_t)stream)
+                             : do_is_cache<float>(s, x, psi, dpsi, d2psi, (hipStream_t)stream);
+}
+
+int pvd_sim_is_propose(pvd_sim* s, const double* disp, double* y, void* stream) {
+  if (!s || !y) return fail(E_BAD, "null argument");
+  return s->prec == PVD_FP64 ? do_is_propose<double>(s, disp, y, (hipStream_t)stream)
+                             : do_is_propose<float>(s, disp, y, (hipStream_t)stream);
+}
+
+int pvd_sim_is_accept(pvd_sim* s, const double* y, const double* psi_y, const double* dpsi_y, const double* d2psi_y,
+                      const double* u_metro, void* stream) {
+  if (!s || !y || !psi_y || !dpsi_y || !d2psi_y) return fail(E_BAD, "null argument");
+  return s->prec == PVD_FP64 ? do_is_accept<double>(s, y, psi_y, dpsi_y, d2psi_y, u_metro, (hipStream_t)stream)
+                             : do_is_accept<float>(s, y, psi_y, dpsi_y, d2psi_y, u_metro, (hipStream_t)stream);
+}
+
+int pvd_sim_pes(pvd_sim* s, double* v, void* stream) {
+  if (!s || !v) return fail(E_BAD, "null argument");
+  return s->prec == PVD_FP64 ? do_pes<double>(s, v, (hipStream_t)stream) : do_pes<float>(s, v, (hipStream_t)stream);
 }
 
 int pvd_sim_planes(pvd_sim* s, void** planes, int64_t* stride, int32_t* n, void* stream) {

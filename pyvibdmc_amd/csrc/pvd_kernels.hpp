@@ -145,6 +145,12 @@ struct StepArgs {
   int branch;
   int ms, ds;  // atoms, dims
   int is_;     // importance sampling on
+  // trial wave function evaluated by the caller: caches of the importance-sampled move as
+  // FP64 planes (see k_his_propose), current and output buffers, plane count (0: none)
+  const double* xc_in;
+  double* xc_out;
+  int nx;
+  int htrial;
 };
 
 // ---------------------------------------------------------------------------- helpers
@@ -810,6 +816,10 @@ __global__ __launch_bounds__(TILE) void k_discrete(StepArgs<R> A) {
           for (int p = 0; p < NP; ++p) A.cout[p * A.cap + dst] = x[k][p];
         }
         if (A.who_out) A.who_out[dst] = A.who_in[i];
+        if constexpr (EXT) {
+          // the trial caches of a caller-evaluated wave function ride along
+          for (int q2 = 0; q2 < A.nx; ++q2) A.xc_out[(long long)q2 * A.cap + dst] = A.xc_in[(long long)q2 * A.cap + i];
+        }
       }
     }
   }
@@ -852,6 +862,31 @@ __global__ __launch_bounds__(TILE) void k_is_move(StepArgs<R> A) {
   if (threadIdx.x == 0 && a) fetch_add_relaxed(acc_slot(A, par, tile), (unsigned)a);
 }
 
+// Displacements of components p0 .. p0+3 of walker i on the split path (run-time component
+// count): injected (the reference's (n, M, d) draws) or Philox blocks tagged 0x80 / 0xC0.
+template <typename R, bool INJ>
+PVD_DEV void split_disp(const StepArgs<R>& A, long long i, unsigned long long step, int p0, R* z) {
+  if constexpr (INJ) {
+    for (int q = 0; q < 4; ++q) z[q] = p0 + q < A.np ? (R)A.inj_disp[i * A.np + p0 + q] : R(0);
+  } else {
+    const Philox4 r = A.rng.block((uint32_t)i, step, RNG_DISP, (uint32_t)(p0 / 4) | 0x80u);
+    if constexpr (sizeof(R) == 4) {
+      box_muller_f(r.v[0], r.v[1], z[0], z[1]);
+      box_muller_f(r.v[2], r.v[3], z[2], z[3]);
+    } else {
+      double a0, a1;
+      box_muller_d(r.v[0], r.v[1], r.v[2], r.v[3], a0, a1);
+      z[0] = a0;
+      z[1] = a1;
+      const Philox4 r2 = A.rng.block((uint32_t)i, step, RNG_DISP, (uint32_t)(p0 / 4) | 0xC0u);
+      box_muller_d(r2.v[0], r2.v[1], r2.v[2], r2.v[3], a0, a1);
+      z[2] = a0;
+      z[3] = a1;
+    }
+    for (int q = 0; q < 4; ++q) z[q] = (R)A.sigma[(p0 + q) < MAXNP ? p0 + q : 0] * z[q];
+  }
+}
+
 // ------------------------------------------------------------------ split path: the move
 // Moves walkers in place (plain or importance-sampled) so that an outside PES can be
 // evaluated on the moved walkers.  IS writes the local kinetic energy per walker and
@@ -884,25 +919,7 @@ __global__ __launch_bounds__(TILE) void k_move(StepArgs<R> A) {
       // runtime component count; draws in blocks of four components
       for (int p0 = 0; p0 < A.np; p0 += 4) {
         R z[4];
-        if constexpr (INJ) {
-          for (int q = 0; q < 4 && p0 + q < A.np; ++q) z[q] = (R)A.inj_disp[i * A.np + p0 + q];
-        } else {
-          const Philox4 r = A.rng.block((uint32_t)i, step, RNG_DISP, (uint32_t)(p0 / 4) | 0x80u);
-          if constexpr (sizeof(R) == 4) {
-            box_muller_f(r.v[0], r.v[1], z[0], z[1]);
-            box_muller_f(r.v[2], r.v[3], z[2], z[3]);
-          } else {
-            double a0, a1;
-            box_muller_d(r.v[0], r.v[1], r.v[2], r.v[3], a0, a1);
-            z[0] = a0;
-            z[1] = a1;
-            const Philox4 r2 = A.rng.block((uint32_t)i, step, RNG_DISP, (uint32_t)(p0 / 4) | 0xC0u);
-            box_muller_d(r2.v[0], r2.v[1], r2.v[2], r2.v[3], a0, a1);
-            z[2] = a0;
-            z[3] = a1;
-          }
-          for (int q = 0; q < 4; ++q) z[q] = (R)A.sigma[(p0 + q) < MAXNP ? p0 + q : 0] * z[q];
-        }
+        split_disp<R, INJ>(A, i, step, p0, z);
         for (int q = 0; q < 4 && p0 + q < A.np; ++q) {
           const long long o = (long long)(p0 + q) * A.cap + i;
           A.cout[o] = A.cin[o] + z[q];
@@ -914,6 +931,153 @@ __global__ __launch_bounds__(TILE) void k_move(StepArgs<R> A) {
     a = block_sum<TILE>(a, s_i);
     if (threadIdx.x == 0 && a) fetch_add_relaxed(acc_slot(A, par, tile), (unsigned)a);
   }
+}
+
+// --------------------------------------------------------------- caller's trial function
+// Importance sampling with a trial wave function evaluated outside the library: the
+// reference's ImpSampManager plugins (call_trial / call_derivs, imp_samp_manager.py:92-139,
+// 197-224) called by imp_move_randomly (pyvibdmc.py:549-612).  The library keeps that
+// move's caches next to the walkers as FP64 planes xc[q * cap + i]:
+//   q in [0, np)      dpsi/psi   (f_x)
+//   q in [np, 2 np)   d2psi/psi  (psi_sec_der)
+//   q = 2 np          psi        (psi_1)
+// and they follow the walkers through branching (pyvibdmc.py:424-427, 353-356).  A step is:
+// k_his_propose (y = x + disp + D_x dt, written for the caller), the caller's psi / derivatives
+// at y, k_his_accept (Metropolis, caches, local kinetic energy), a potential, branching.
+template <typename R>
+PVD_DEV double his_local_kin(const StepArgs<R>& A, const double* sd, long long stride, long long i) {
+  // -0.5 sum_c sum_a inv_m sd, atoms summed first (imp_samp.py:49-53)
+  double tot = 0.0;
+  for (int c = 0; c < A.ds; ++c) {
+    double s = 0.0;
+    for (int a = 0; a < A.ms; ++a) {
+      const int p = a * A.ds + c;
+      const double v = A.inv_m[p] * sd[p * stride + i];
+      s = (a == 0) ? v : s + v;
+    }
+    tot = (c == 0) ? s : tot + s;
+  }
+  return -0.5 * tot;
+}
+
+// The proposal of pyvibdmc.py:556-564: y = (x + disp) + (D_x dt), D_x = inv_m * f_x, as
+// (n, M, d) float64 rows for the caller.
+template <typename R, bool INJ>
+__global__ __launch_bounds__(TILE) void k_his_propose(StepArgs<R> A, double* y) {
+  const int par = A.par;
+  const Ctrl* C = A.ctrl;
+  const int n = C->n[par];
+  const long long i = (long long)blockIdx.x * TILE + threadIdx.x;
+  if (C->status != ST_OK || i >= n) return;
+  const unsigned long long step = C->step[par];
+  const int np = A.np;
+  for (int p0 = 0; p0 < np; p0 += 4) {
+    R z[4];
+    split_disp<R, INJ>(A, i, step, p0, z);
+    for (int q = 0; q < 4 && p0 + q < np; ++q) {
+      const int p = p0 + q;
+      const double dx = A.inv_m[p] * A.xc_in[(long long)p * A.cap + i];
+      y[i * np + p] = ((double)A.cin[(long long)p * A.cap + i] + (double)z[q]) + dx * A.dt;
+    }
+  }
+}
+
+// Metropolis test of pyvibdmc.py:566-605 in the literal form of imp_samp.py:28-47 (FP64:
+// the ratio of psi squared times, per component, exp(-(x - y - D_y dt)^2 / 2 sigma^2) over
+// exp(-(y - x - D_x dt)^2 / 2 sigma^2), multiplied over atoms and then over dimensions; zero
+// when psi changes sign), then the accepted walkers take y and its caches
+// (pyvibdmc.py:603-609) and every walker stores the local kinetic energy of its final
+// position for the branching kernel (pyvibdmc.py:807-809).  y is the caller's array after
+// its trial call (which may have perturbed it in place, as finite differences do).
+template <typename R, bool INJ>
+__global__ __launch_bounds__(TILE) void k_his_accept(StepArgs<R> A, const double* y, const double* psi_y,
+                                                     const double* f_y, const double* sd_y) {
+  __shared__ int s_i[TILE / PVD_WAVE];
+  const int par = A.par;
+  const Ctrl* C = A.ctrl;
+  const int n = C->n[par];
+  const int ntiles = (n + TILE - 1) / TILE;
+  const int tile = blockIdx.x;
+  if (C->status != ST_OK || tile >= ntiles) return;
+  const unsigned long long step = C->step[par];
+  const long long i = (long long)tile * TILE + threadIdx.x;
+  const int np = A.np;
+  const long long cap = A.cap;
+  int a = 0;
+  if (i < n) {
+    const double u = INJ ? A.inj_umet[i] : uniform_d(A.rng, (uint32_t)i, step, RNG_METRO);
+    const double psx = A.xc_in[2ll * np * cap + i], psy = psi_y[i];
+    const double r = psy / psx;
+    const double ratio = r * r;
+    double prod = 1.0;
+    for (int c = 0; c < A.ds; ++c) {
+      double pc = 1.0;
+      for (int at = 0; at < A.ms; ++at) {
+        const int p = at * A.ds + c;
+        const double s2 = A.sigma[p] * A.sigma[p];
+        const double xp = (double)A.cin[(long long)p * cap + i], yp = y[i * np + p];
+        const double a1 = (xp - yp) - (A.inv_m[p] * f_y[i * np + p]) * A.dt;
+        const double a2 = (yp - xp) - (A.inv_m[p] * A.xc_in[(long long)p * cap + i]) * A.dt;
+        const double t1 = exp((-1.0 * (a1 * a1)) / (2.0 * s2));
+        const double t2 = exp((-1.0 * (a2 * a2)) / (2.0 * s2));
+        const double tc = t1 / t2;
+        pc = (at == 0) ? tc : pc * tc;
+      }
+      prod = (c == 0) ? pc : prod * pc;
+    }
+    double met = prod * ratio;
+    if (psx * psy <= 0.0) met = 0.0;
+    const bool acc = met > u;
+    a = acc ? 1 : 0;
+    R* x = const_cast<R*>(A.cin);
+    double* xc = const_cast<double*>(A.xc_in);
+    if (acc) {
+      for (int p = 0; p < np; ++p) {
+        x[(long long)p * cap + i] = (R)y[i * np + p];
+        xc[(long long)p * cap + i] = f_y[i * np + p];
+        xc[(long long)(np + p) * cap + i] = sd_y[i * np + p];
+      }
+      xc[2ll * np * cap + i] = psy;
+    }
+    A.ke[i] = his_local_kin(A, xc + (long long)np * cap, cap, i);
+  }
+  a = block_sum<TILE>(a, s_i);
+  if (threadIdx.x == 0 && a) fetch_add_relaxed(acc_slot(A, par, tile), (unsigned)a);
+}
+
+// Caches of the current walkers from the caller's trial call (the first step's
+// self.impsamp.drift(self._walker_coords), pyvibdmc.py:552-553, 763-767): rows (n, M, d)
+// float64; x (optional) replaces the coordinates (a finite-difference call perturbs them in
+// place), and the local kinetic energy of sd is stored for the first-step energies.
+template <typename R>
+__global__ __launch_bounds__(TILE) void k_his_cache(StepArgs<R> A, const double* x, const double* psi,
+                                                    const double* f, const double* sd) {
+  const int n = A.ctrl->n[A.par];
+  const long long i = (long long)blockIdx.x * TILE + threadIdx.x;
+  if (i >= n) return;
+  const int np = A.np;
+  const long long cap = A.cap;
+  R* xo = const_cast<R*>(A.cin);
+  double* xc = const_cast<double*>(A.xc_in);
+  for (int p = 0; p < np; ++p) {
+    if (x) xo[(long long)p * cap + i] = (R)x[i * np + p];
+    xc[(long long)p * cap + i] = f[i * np + p];
+    xc[(long long)(np + p) * cap + i] = sd[i * np + p];
+  }
+  xc[2ll * np * cap + i] = psi[i];
+  A.ke[i] = his_local_kin(A, xc + (long long)np * cap, cap, i);
+}
+
+// The simulation's device potential on its current walkers (potential.getpot on
+// self._walker_coords, pyvibdmc.py:786-793), float64 energies.
+template <typename R, int NP>
+__global__ __launch_bounds__(TILE) void k_pes_planes(StepArgs<R> A, double* v) {
+  const int n = A.ctrl->n[A.par];
+  const long long i = (long long)blockIdx.x * TILE + threadIdx.x;
+  if (i >= n) return;
+  R x[NP];
+  load_walker<R, NP>(A, i, x);
+  v[i] = (double)pes_eval<R, NP>(A.pes, x);
 }
 
 // ------------------------------------------------------------------------- continuous
@@ -1925,6 +2089,7 @@ __global__ __launch_bounds__(RP_NT) void k_cont_replay(StepArgs<R> A) {
   // their slot's own turn are dropped)
   const int nfin = s_nfin;
   const int np = A.np;
+  const long long rsw = np + 2 + A.nx;  // staged doubles per copy
   double lsw = 0.0, lswe = 0.0;
   for (int q = tid; q < nfin; q += RP_NT) {
     Piece pc = A.fin[q];
@@ -1939,9 +2104,11 @@ __global__ __launch_bounds__(RP_NT) void k_cont_replay(StepArgs<R> A) {
     lsw += pc.v - wold;
     lswe += pc.v * esrc - wold * A.E[s];
     if (s != pc.src) {
-      for (int p = 0; p < np; ++p) A.rstage[(long long)q * (np + 2) + p] = (double)A.cout[p * A.cap + pc.src];
-      A.rstage[(long long)q * (np + 2) + np] = esrc;
-      A.rstage[(long long)q * (np + 2) + np + 1] = A.who_out ? (double)A.who_out[pc.src] : 0.0;
+      double* st = A.rstage + (long long)q * rsw;
+      for (int p = 0; p < np; ++p) st[p] = (double)A.cout[p * A.cap + pc.src];
+      st[np] = esrc;
+      st[np + 1] = A.who_out ? (double)A.who_out[pc.src] : 0.0;
+      for (int p = 0; p < A.nx; ++p) st[np + 2 + p] = A.xc_in[(long long)p * A.cap + pc.src];
     }
   }
   lsw = block_sum<RP_NT>(lsw, s_d);
@@ -1951,9 +2118,11 @@ __global__ __launch_bounds__(RP_NT) void k_cont_replay(StepArgs<R> A) {
     if (pc.slot < 0) continue;
     A.w[pc.slot] = pc.v;
     if (pc.slot != pc.src) {
-      for (int p = 0; p < np; ++p) A.cout[p * A.cap + pc.slot] = (R)A.rstage[(long long)q * (np + 2) + p];
-      A.E[pc.slot] = A.rstage[(long long)q * (np + 2) + np];
-      if (A.who_out) A.who_out[pc.slot] = (int)A.rstage[(long long)q * (np + 2) + np + 1];
+      const double* st = A.rstage + (long long)q * rsw;
+      for (int p = 0; p < np; ++p) A.cout[p * A.cap + pc.slot] = (R)st[p];
+      A.E[pc.slot] = st[np];
+      if (A.who_out) A.who_out[pc.slot] = (int)st[np + 1];
+      for (int p = 0; p < A.nx; ++p) A.xc_out[(long long)p * A.cap + pc.slot] = st[np + 2 + p];
     }
   }
   // the next threshold from every weight after the split (this path is the exception)
@@ -1995,6 +2164,7 @@ __global__ __launch_bounds__(TILE) void k_cont_copy(StepArgs<R> A) {
   }
   A.E[ks] = e;
   if (A.who_out) A.who_out[ks] = who;
+  for (int q = 0; q < A.nx; ++q) A.xc_out[(long long)q * A.cap + ks] = A.xc_in[(long long)q * A.cap + o];
 }
 
 }  // namespace pvd

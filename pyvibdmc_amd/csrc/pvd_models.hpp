@@ -270,6 +270,7 @@ enum TrialKind : int {
   TRIAL_NONE = 0,
   TRIAL_HARMONIC_1D = 1,  // psi = c exp(-alpha x^2/2) on coordinate 0
   TRIAL_GAUSS = 2,        // product of Gaussians in bond lengths and bond angles
+  TRIAL_HOST = 3,         // evaluated by the caller (split importance-sampled move)
 };
 
 #define PVD_MAX_TB 8

@@ -41,6 +41,9 @@ inline unsigned nblk(long long n, int bs = 256) { return (unsigned)((n + bs - 1)
 struct pvd_sim {
   pvd_config cfg;
   int np, M, d, prec, cont, is;
+  int htrial;      // trial wave function evaluated by the caller
+  int nx;          // its cache planes (2 np + 1), 0 without one
+  double* xc[2];   // the cache planes, by step parity like the coordinates
   int mspec;  // ModelSpec of the fused step kernels (0 = runtime model parameters)
   long long cap;
   void* coords[2];
@@ -162,6 +165,12 @@ StepArgs<R> make_args(pvd_sim* s) {
   A.ms = s->M;
   A.ds = s->d;
   A.is_ = s->is;
+  if (s->htrial) {
+    A.xc_in = s->xc[cb];
+    A.xc_out = s->cont ? s->xc[0] : s->xc[cb ^ 1];
+    A.nx = s->nx;
+    A.htrial = 1;
+  }
   return A;
 }
 
@@ -222,12 +231,27 @@ template <typename R>
 int do_branch_ext(pvd_sim* s, const double* v, const double* ub, hipStream_t st);
 template <typename R>
 int do_start(pvd_sim* s, const double* v, hipStream_t st);
+template <typename R>
+int do_is_cache(pvd_sim* s, const double* x, const double* psi, const double* f, const double* sd, hipStream_t st);
+template <typename R>
+int do_is_propose(pvd_sim* s, const double* disp, double* y, hipStream_t st);
+template <typename R>
+int do_is_accept(pvd_sim* s, const double* y, const double* psi, const double* f, const double* sd,
+                 const double* um, hipStream_t st);
+template <typename R>
+int do_pes(pvd_sim* s, double* v, hipStream_t st);
 
 #define PVD_SIM_OPS(R)                                                                                   \
   template int run_steps<R>(pvd_sim*, int, const double*, const double*, const double*, hipStream_t);  \
   template int run_steps_timed<R>(pvd_sim*, int, hipStream_t, float*, int32_t*);                       \
   template int do_move<R>(pvd_sim*, const double*, const double*, hipStream_t);                         \
   template int do_branch_ext<R>(pvd_sim*, const double*, const double*, hipStream_t);                   \
-  template int do_start<R>(pvd_sim*, const double*, hipStream_t);
+  template int do_start<R>(pvd_sim*, const double*, hipStream_t);                                      \
+  template int do_is_cache<R>(pvd_sim*, const double*, const double*, const double*, const double*,     \
+                              hipStream_t);                                                              \
+  template int do_is_propose<R>(pvd_sim*, const double*, double*, hipStream_t);                         \
+  template int do_is_accept<R>(pvd_sim*, const double*, const double*, const double*, const double*,    \
+                               const double*, hipStream_t);                                              \
+  template int do_pes<R>(pvd_sim*, double*, hipStream_t);
 
 }  // namespace pvd

@@ -23,6 +23,7 @@ __global__ __launch_bounds__(TILE) void k_start(StepArgs<R> A, int continuous) {
   if (i < n) {
     if constexpr (EXT) {
       E = A.V_ext[i];
+      if (A.htrial) E = E + A.ke[i];  // local kinetic energy from pvd_sim_is_cache
     }
     if constexpr (!EXT || IS) {
       R x[NP];
@@ -215,6 +216,7 @@ int surrogate_step(pvd_sim* s, hipStream_t st) {
 template <typename R>
 int run_steps(pvd_sim* s, int n_steps, const double* disp, const double* um, const double* ub,
               hipStream_t st) {
+  if (s->htrial) return fail(E_BAD, "a caller-evaluated trial runs the split step (is_propose / is_accept)");
   if (s->surr_on) {
     if (disp) return fail(E_BAD, "injected draws with a surrogate potential: use move / branch_external");
     for (int k = 0; k < n_steps; ++k) {
@@ -266,6 +268,7 @@ int run_steps(pvd_sim* s, int n_steps, const double* disp, const double* um, con
 // Eager steps with an event pair on every kernel dispatch (mean duration per kernel slot).
 template <typename R>
 int run_steps_timed(pvd_sim* s, int n_steps, hipStream_t st, float* avg_ms, int32_t* n_kernels) {
+  if (s->htrial) return fail(E_BAD, "a caller-evaluated trial runs the split step (is_propose / is_accept)");
   Launch L[8];
   int nl = 0;
   StepArgs<R> A0 = make_args<R>(s);
@@ -316,6 +319,7 @@ int run_steps_timed(pvd_sim* s, int n_steps, hipStream_t st, float* avg_ms, int3
 
 template <typename R>
 int do_move(pvd_sim* s, const double* disp, const double* um, hipStream_t st) {
+  if (s->htrial) return fail(E_BAD, "a caller-evaluated trial moves with is_propose / is_accept");
   StepArgs<R> A = make_args<R>(s);
   // the move happens in place in the current buffer
   A.cout = (R*)A.cin;
@@ -378,8 +382,9 @@ int do_start(pvd_sim* s, const double* v, hipStream_t st) {
   StepArgs<R> A = make_args<R>(s);
   A.V_ext = v;
   const unsigned g = nblk(s->cap, TILE);
+  if (s->htrial && !v) return fail(E_BAD, "a caller-evaluated trial needs the first-step energies (start_external)");
   if (v) {
-    if (s->is) {
+    if (s->is && !s->htrial) {
       PVD_NP_SWITCH(s->np, hipLaunchKernelGGL((k_start<R, NP, true, true>), dim3(g), dim3(TILE), 0, st, A, s->cont));
     } else {
       hipLaunchKernelGGL((k_start<R, 1, false, true>), dim3(g), dim3(TILE), 0, st, A, s->cont);
@@ -391,6 +396,47 @@ int do_start(pvd_sim* s, const double* v, hipStream_t st) {
       PVD_NP_SWITCH(s->np, hipLaunchKernelGGL((k_start<R, NP, false, false>), dim3(g), dim3(TILE), 0, st, A, s->cont));
     }
   }
+  HIPCHK(hipGetLastError());
+  return 0;
+}
+
+template <typename R>
+int do_is_cache(pvd_sim* s, const double* x, const double* psi, const double* f, const double* sd, hipStream_t st) {
+  if (!s->htrial) return fail(E_BAD, "the simulation was not created with a caller-evaluated trial");
+  StepArgs<R> A = make_args<R>(s);
+  hipLaunchKernelGGL((k_his_cache<R>), dim3(nblk(s->cap, TILE)), dim3(TILE), 0, st, A, x, psi, f, sd);
+  HIPCHK(hipGetLastError());
+  return 0;
+}
+
+template <typename R>
+int do_is_propose(pvd_sim* s, const double* disp, double* y, hipStream_t st) {
+  if (!s->htrial) return fail(E_BAD, "the simulation was not created with a caller-evaluated trial");
+  StepArgs<R> A = make_args<R>(s);
+  A.inj_disp = disp;
+  if (disp) hipLaunchKernelGGL((k_his_propose<R, true>), dim3(nblk(s->cap, TILE)), dim3(TILE), 0, st, A, y);
+  else hipLaunchKernelGGL((k_his_propose<R, false>), dim3(nblk(s->cap, TILE)), dim3(TILE), 0, st, A, y);
+  HIPCHK(hipGetLastError());
+  return 0;
+}
+
+template <typename R>
+int do_is_accept(pvd_sim* s, const double* y, const double* psi, const double* f, const double* sd,
+                 const double* um, hipStream_t st) {
+  if (!s->htrial) return fail(E_BAD, "the simulation was not created with a caller-evaluated trial");
+  StepArgs<R> A = make_args<R>(s);
+  A.inj_umet = um;
+  if (um) hipLaunchKernelGGL((k_his_accept<R, true>), dim3(nblk(s->cap, TILE)), dim3(TILE), 0, st, A, y, psi, f, sd);
+  else hipLaunchKernelGGL((k_his_accept<R, false>), dim3(nblk(s->cap, TILE)), dim3(TILE), 0, st, A, y, psi, f, sd);
+  HIPCHK(hipGetLastError());
+  return 0;
+}
+
+template <typename R>
+int do_pes(pvd_sim* s, double* v, hipStream_t st) {
+  if (s->cfg.pes.kind == PES_EXTERNAL) return fail(E_BAD, "the simulation has no device potential");
+  StepArgs<R> A = make_args<R>(s);
+  PVD_NP_SWITCH(s->np, hipLaunchKernelGGL((k_pes_planes<R, NP>), dim3(nblk(s->cap, TILE)), dim3(TILE), 0, st, A, v));
   HIPCHK(hipGetLastError());
   return 0;
 }

@@ -113,7 +113,9 @@ class DMC_Sim:
         self._deb_alpha = DEBUG_alpha
         self.precision = precision.lower()
         self.rng = rng
-        self.seed = int(np.random.randint(0, 2 ** 62)) if seed is None else int(seed)
+        # the default Philox key comes from the OS, not from np.random: the global NumPy
+        # stream stays the reference's own (rng='numpy' draws from it)
+        self.seed = int.from_bytes(os.urandom(8), 'little') >> 2 if seed is None else int(seed)
         self.write_files = write_files
         self.capacity = capacity
         self._initialize()
@@ -202,12 +204,19 @@ class DMC_Sim:
         lib = N.load()
         pes = getattr(self.potential_obj, '_device_pes', None)
         trial = None
+        # a trial wave function without a device form (the reference's ImpSampManager with
+        # user Python functions) is called on the host around a device move and Metropolis test
+        self._htrial = False
         if self.impsamp_manager is not None:
             trial = getattr(self.impsamp_manager, '_device_trial', None)
             if trial is None:
-                raise NotImplementedError('importance sampling on the GPU needs a device trial wave function '
-                                          '(pyvibdmc_amd.GaussianProductTrial or HarmonicTrial1D)')
-        self._fused = pes is not None and pes.kind != N.PES_EXTERNAL
+                from .impsamp import ImpSamp
+                self._htrial = True
+                self.impsamp = ImpSamp(self.impsamp_manager)
+                trial = N.TrialDesc()
+                trial.kind = N.TRIAL_HOST
+        self._dev_pes = pes is not None and pes.kind != N.PES_EXTERNAL
+        self._fused = self._dev_pes and not self._htrial
         n, m, d = coords.shape
         np_ = m * d
         cfg = N.Config()
@@ -353,10 +362,77 @@ class DMC_Sim:
         v = self.potential(coords)
         return np.ascontiguousarray(np.asarray(v, dtype=np.float64).reshape(-1))
 
+    def _energy_dev(self):
+        """V of the current walkers as a device float64 tensor: the device potential in place,
+        a device-energy object, or the host potential on a host copy (pyvibdmc.py:786-793)."""
+        import torch
+        if self._dev_pes:
+            v = torch.empty(max(self._n_host, 1), dtype=torch.float64, device='cuda')
+            N.check(N.load().pvd_sim_pes(self._h, N.ptr(v), N.stream_handle()), 'pvd_sim_pes')
+            return v
+        dev_e = getattr(self.potential_obj, 'device_energy', None)
+        if dev_e is not None:
+            return dev_e(self)
+        c, _ = self._get_walkers()
+        return self._dev(self._host_energy(c))
+
+    def _drift_to_device(self, cds):
+        """ImpSamp.drift on host walkers (imp_samp.py:21-26) -> device (psi, dpsi/psi, d2psi/psi)."""
+        f, psi, sd = self.impsamp.drift(cds)
+        n = len(cds)
+        shape = (n,) + self._shape
+        return (self._dev(np.asarray(psi, dtype=np.float64).reshape(n)),
+                self._dev(np.broadcast_to(np.asarray(f, dtype=np.float64), shape)),
+                self._dev(np.broadcast_to(np.asarray(sd, dtype=np.float64), shape)))
+
+    def _start_host_trial(self):
+        """First step with a host trial (pyvibdmc.py:760-769, 552-553): V(x0), the local kinetic
+        energy of drift(x0), vref; then imp_move_randomly's own drift of the (possibly perturbed)
+        walkers fills the caches.  The walkers go back to the device after each drift, since a
+        finite-difference trial call perturbs its input in place (imp_samp.py:67-71)."""
+        lib = N.load()
+        st = N.stream_handle()
+        v = self._energy_dev()
+        for first in (True, False):
+            c, _ = self._get_walkers()
+            psi, f, sd = self._drift_to_device(c)
+            N.check(lib.pvd_sim_is_cache(self._h, N.ptr(self._dev(c)), N.ptr(psi), N.ptr(f), N.ptr(sd), st),
+                    'pvd_sim_is_cache')
+            if first:
+                N.check(lib.pvd_sim_start_external(self._h, N.ptr(v), st), 'pvd_sim_start_external')
+        self._keep = v
+
+    def _host_trial_step(self, t, draws):
+        """One importance-sampled step with a host trial (pyvibdmc.py:549-612, 786-848): device
+        proposal, the trial and its derivatives at the proposal on the host, device Metropolis,
+        potential, device branching."""
+        import torch
+        lib = N.load()
+        st = N.stream_handle()
+        n = self._n_host
+        m, d = self._shape
+        disp = self._dev(draws.displacement(self._sigmas, (n, m, d))) if draws is not None else None
+        y = torch.empty(n * m * d, dtype=torch.float64, device='cuda')
+        N.check(lib.pvd_sim_is_propose(self._h, N.ptr(disp), N.ptr(y), st), 'pvd_sim_is_propose')
+        yh = y.cpu().numpy().reshape(n, m, d)
+        psi, f, sd = self._drift_to_device(yh)
+        y.copy_(torch.from_numpy(yh.reshape(-1)))
+        um = self._dev(draws.uniform(n)) if draws is not None else None
+        N.check(lib.pvd_sim_is_accept(self._h, N.ptr(y), N.ptr(psi), N.ptr(f), N.ptr(sd), N.ptr(um), st),
+                'pvd_sim_is_accept')
+        v = self._energy_dev()
+        ub = None
+        if draws is not None and self.weighting == 'discrete' and t in self._branch_step:
+            ub = self._dev(draws.uniform(n))
+        N.check(lib.pvd_sim_branch_external(self._h, N.ptr(v), N.ptr(ub), st), 'pvd_sim_branch_external')
+        self._raise_if_failed()
+
     def _start(self):
         lib = N.load()
         dev_e = getattr(self.potential_obj, 'device_energy', None)
-        if self._fused or self._surrogate:
+        if self._htrial:
+            self._start_host_trial()
+        elif self._fused or self._surrogate:
             N.check(lib.pvd_sim_start(self._h, N.stream_handle()), 'pvd_sim_start')
         elif dev_e is not None:
             v = dev_e(self)
@@ -406,7 +482,10 @@ class DMC_Sim:
         draws = self._draws()
         dev_e = getattr(self.potential_obj, 'device_energy', None)
         lib = N.load()
-        if draws is None and (self._fused or self._surrogate):
+        if self._htrial:
+            for t in range(t0, t1):
+                self._host_trial_step(t, draws)
+        elif draws is None and (self._fused or self._surrogate):
             N.check(lib.pvd_sim_run(self._h, t1 - t0, N.stream_handle()), 'pvd_sim_run')
             self._raise_if_failed()
         elif draws is None and dev_e is not None:
@@ -541,8 +620,8 @@ class DMC_Sim:
     def _state(self):
         c, w = self._get_walkers()
         keep = {k: v for k, v in self.__dict__.items()
-                if k not in ('potential', 'potential_obj', 'potential_info', 'impsamp_manager', '_h', '_cfg',
-                             '_logger', '_keep', 'rng')}
+                if k not in ('potential', 'potential_obj', 'potential_info', 'impsamp_manager', 'impsamp', '_h',
+                             '_cfg', '_logger', '_keep', 'rng')}
         keep = copy.deepcopy(keep)
         keep['_saved_coords'] = c
         keep['_saved_wts'] = w
