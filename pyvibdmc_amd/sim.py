@@ -382,8 +382,8 @@ class DMC_Sim:
         n = len(cds)
         shape = (n,) + self._shape
         return (self._dev(np.asarray(psi, dtype=np.float64).reshape(n)),
-                self._dev(np.broadcast_to(np.asarray(f, dtype=np.float64), shape)),
-                self._dev(np.broadcast_to(np.asarray(sd, dtype=np.float64), shape)))
+                self._dev(np.asarray(f, dtype=np.float64).reshape(shape)),
+                self._dev(np.asarray(sd, dtype=np.float64).reshape(shape)))
 
     def _start_host_trial(self):
         """First step with a host trial (pyvibdmc.py:760-769, 552-553): V(x0), the local kinetic
