@@ -249,6 +249,16 @@ int pvd_mlp_judge(const pvd_pes_desc* pes, int32_t precision, int32_t n_atoms, c
  * x_t (D x n) and targets (n); y_out (n) or NULL; loss_parts (ceil(n/256) doubles). */
 int pvd_mlp_grads(const pvd_mlp* m, const float* x_t, const float* targets, int64_t n, float* grads, float* y_out,
                   double* loss_parts, void* workspace, void* stream);
+/* pvd_mlp_grads without a workspace argument and without any device allocation.  No
+ * counterpart in the reference: it is for callers that cannot allocate device memory (code
+ * under HIP graph capture, training loops that forbid hipMalloc).  The workspace is a static
+ * device array of pvd_mlp_noalloc_scratch_bytes() (32 MiB) per device, held until the process
+ * ends; batches whose workspace (pvd_mlp_workspace_bytes) exceeds it are rejected with code 7.
+ * Every call on one device shares that device's array: calls that overlap on one GPU race.
+ * Nothing in it needs clearing (each call writes what it reads). */
+int64_t pvd_mlp_noalloc_scratch_bytes(void);
+int pvd_mlp_grads_noalloc(const pvd_mlp* m, const float* x_t, const float* targets, int64_t n, float* grads,
+                          float* y_out, double* loss_parts, void* stream);
 /* One Adam step (Kingma & Ba) with bias correction at step t >= 1. */
 int pvd_mlp_adam(float* params, float* m1, float* m2, const float* grads, int64_t size, float lr, float beta1,
                  float beta2, float eps, int64_t t, void* stream);

@@ -11,6 +11,7 @@
 // makes those registers the next B operand directly, so activations never leave
 // registers; the weights are stored pre-permuted to match (pvd_mlp_pack).
 #include <hip/hip_runtime.h>
+#include <mutex>
 
 #include <algorithm>
 #include <cstring>
@@ -1074,8 +1075,70 @@ int64_t pvd_mlp_workspace_bytes(int32_t D, int32_t H, int64_t batch) {
   return (3 * H * n * 2 + 2 * n + P + (n + 255) / 256 * 2) * 4 + 64;
 }
 
+}  // extern "C"
+
+namespace {
+
+// Scratch of pvd_mlp_grads_noalloc: one array per device, held until the process ends.
+// Host code never uses the name `scratch` as an address: the device address comes from
+// hipGetSymbolAddress, looked up once per device with that device current.
+constexpr long long PVD_MLP_SCRATCH_BYTES = 32ll << 20;
+static __device__ __align__(256) unsigned char scratch[PVD_MLP_SCRATCH_BYTES];
+
+constexpr int MAX_DEVICES = 64;
+std::mutex g_scratch_mu;
+void* g_scratch[MAX_DEVICES];
+
+int scratch_for_current_device(void** out) {
+  int dev = 0;
+  MHIP(hipGetDevice(&dev));
+  if (dev < 0 || dev >= MAX_DEVICES) return mfail(7, "device index beyond the scratch table");
+  std::lock_guard<std::mutex> lock(g_scratch_mu);
+  if (!g_scratch[dev]) {
+    void* ptr = nullptr;
+    const hipError_t e = hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch));
+    if (e != hipSuccess || !ptr) return mfail(8, std::string("hipGetSymbolAddress(scratch): ") + hipGetErrorString(e));
+    g_scratch[dev] = ptr;
+  }
+  *out = g_scratch[dev];
+  return 0;
+}
+
+int mlp_grads(const pvd_mlp* m, const float* x_t, const float* targets, int64_t n, float* grads, float* y_out,
+              double* loss_parts, void* workspace, void* stream);
+
+}  // namespace
+
+extern "C" {
+
 int pvd_mlp_grads(const pvd_mlp* m, const float* x_t, const float* targets, int64_t n, float* grads, float* y_out,
                   double* loss_parts, void* workspace, void* stream) {
+  if (!workspace) return mfail(7, "bad arguments");
+  return mlp_grads(m, x_t, targets, n, grads, y_out, loss_parts, workspace, stream);
+}
+
+int64_t pvd_mlp_noalloc_scratch_bytes(void) { return PVD_MLP_SCRATCH_BYTES; }
+
+int pvd_mlp_grads_noalloc(const pvd_mlp* m, const float* x_t, const float* targets, int64_t n, float* grads,
+                          float* y_out, double* loss_parts, void* stream) {
+  if (!m) return mfail(7, "bad arguments");
+  if (n <= 0) return mfail(7, "empty batch");
+  const int64_t need = pvd_mlp_workspace_bytes(m->D, m->H, n);
+  if (need > PVD_MLP_SCRATCH_BYTES)
+    return mfail(7, "batch too large for the static scratch: needs " + std::to_string(need) + " bytes, the maximum is " +
+                        std::to_string(PVD_MLP_SCRATCH_BYTES));
+  void* ws = nullptr;
+  const int rc = scratch_for_current_device(&ws);
+  if (rc) return rc;
+  return mlp_grads(m, x_t, targets, n, grads, y_out, loss_parts, ws, stream);
+}
+
+}  // extern "C"
+
+namespace {
+
+int mlp_grads(const pvd_mlp* m, const float* x_t, const float* targets, int64_t n, float* grads, float* y_out,
+              double* loss_parts, void* workspace, void* stream) {
   MlpNet N;
   if (!m || !x_t || !targets || !grads || !workspace || !build_net(m, N)) return mfail(7, "bad arguments");
   if (n <= 0) return mfail(7, "empty batch");
@@ -1130,6 +1193,10 @@ int pvd_mlp_grads(const pvd_mlp* m, const float* x_t, const float* targets, int6
   MHIP(hipGetLastError());
   return 0;
 }
+
+}  // namespace
+
+extern "C" {
 
 int pvd_mlp_adam(float* params, float* m1, float* m2, const float* grads, int64_t size, float lr, float beta1,
                  float beta2, float eps, int64_t t, void* stream) {

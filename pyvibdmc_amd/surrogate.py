@@ -42,6 +42,8 @@ _MSIGS = {
     'pvd_mlp_energy': (ctypes.c_int, [ctypes.POINTER(MLPDesc), ctypes.c_int32, ctypes.c_int32, _P, ctypes.c_int64,
                                       _P, ctypes.c_int64, _P, ctypes.POINTER(N.PesDesc), _P]),
     'pvd_mlp_grads': (ctypes.c_int, [ctypes.POINTER(MLPDesc), _P, _P, ctypes.c_int64, _P, _P, _P, _P, _P]),
+    'pvd_mlp_grads_noalloc': (ctypes.c_int, [ctypes.POINTER(MLPDesc), _P, _P, ctypes.c_int64, _P, _P, _P, _P]),
+    'pvd_mlp_noalloc_scratch_bytes': (ctypes.c_int64, []),
     'pvd_mlp_adam': (ctypes.c_int, [_P, _P, _P, _P, ctypes.c_int64, ctypes.c_float, ctypes.c_float,
                                     ctypes.c_float, ctypes.c_float, ctypes.c_int64, _P]),
     'pvd_sim_device_view': (ctypes.c_int, [_P, ctypes.POINTER(_P), ctypes.POINTER(ctypes.c_int64),
