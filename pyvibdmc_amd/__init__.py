@@ -3,6 +3,7 @@
 Drop-in names of the reference package (``import pyvibdmc as pv`` scripts keep working with
 ``import pyvibdmc_amd as pv``): DMC_Sim, dmc_restart, Potential, Potential_NoMP,
 NN_Potential, Potential_Direct, ImpSampManager, ImpSampManager_NoMP, Constants.
+Results: SimInfo (the reference's reader of the HDF5 outputs).
 Device potentials and trial wave functions (fused into the step kernels):
 HarmonicOscillator1D, MorseOscillator1D, MorseBendTriatomic, TwoMonomerCluster,
 HarmonicTrial1D, GaussianProductTrial.  Surrogate potential: SurrogatePotential.
@@ -15,5 +16,7 @@ from .impsamp import (ImpSamp, ImpSampManager, ImpSampManager_NoMP, DeviceTrial,
 from .sim import DMC_Sim, dmc_restart
 from .surrogate import SurrogateModel, SurrogatePotential, pair_groups
 from ._native import DMCError, load as load_library
+from .analysis import SimInfo
+from .hdf5 import save_h5
 
 __version__ = '0.1.0'

@@ -23,6 +23,7 @@ import time
 import numpy as np
 
 from . import _native as N
+from . import hdf5
 from .constants import Constants, get_atomic_num
 
 __all__ = ['DMC_Sim', 'dmc_restart']
@@ -554,13 +555,15 @@ class DMC_Sim:
         self.wfns = getattr(self, 'wfns', [])
         self.wfns.append((start_step, rec))
         if self.write_files:
-            np.savez(f"{self.output_folder}/wfns/{self.sim_name}_wfn_{start_step}ts.npz", **rec)
+            # SimArchivist.save_h5 of pyvibdmc.py:861-869
+            hdf5.write(f"{self.output_folder}/wfns/{self.sim_name}_wfn_{start_step}ts.hdf5", list(rec), list(rec.values()))
 
     def _save_training(self, step):
         c, _ = self._get_walkers()
         pots = np.asarray(self.potential(c), dtype=np.float64)
         if self.write_files:
-            np.savez(f"{self.output_folder}/{self.sim_name}_training_{step}ts.npz", coords=c, pots=pots)
+            # pyvibdmc.py:800-804, 840-844
+            hdf5.write(f"{self.output_folder}/{self.sim_name}_training_{step}ts.hdf5", ['coords', 'pots'], [c, pots])
         self.training = getattr(self, 'training', [])
         self.training.append((step, c, pots))
 
@@ -591,7 +594,9 @@ class DMC_Sim:
                                  pop_vs_tau=np.column_stack((ts, self._pop_vs_tau)),
                                  atomic_nums=np.asarray(self._atm_nums), atomic_masses=np.asarray(self.masses))
             if self.write_files:
-                np.savez(f"{self.output_folder}/{self.sim_name}_sim_info.npz", **self.sim_info)
+                # pyvibdmc.py:918-922
+                hdf5.write(f"{self.output_folder}/{self.sim_name}_sim_info.hdf5", list(self.sim_info),
+                           list(self.sim_info.values()))
             self.run_time = time.time() - start
             self._logger.write(f"Simulation took {self.run_time} seconds.\n\n")
             self._logger.close()

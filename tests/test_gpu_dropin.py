@@ -238,7 +238,17 @@ def test_run_writes_results_and_checkpoints(tmp_path, capsys):
     info = sim.sim_info
     assert info['vref_vs_tau'].shape == (300, 2)
     np.testing.assert_array_equal(info['vref_vs_tau'][:, 0], np.arange(300) * 10.0)
-    assert len(os.listdir(tmp_path / 'wfns')) == 2
+    assert sorted(os.listdir(tmp_path / 'wfns')) == ['ho_wfn_100ts.hdf5', 'ho_wfn_200ts.hdf5']
+    # the reference's reader sees what run() saved (pyvibdmc.py:861-869, 918-922)
+    si = pv.SimInfo(str(tmp_path / 'ho_sim_info.hdf5'))
+    np.testing.assert_array_equal(si.get_vref(), info['vref_vs_tau'])
+    np.testing.assert_array_equal(si.get_pop(), info['pop_vs_tau'])
+    np.testing.assert_array_equal(si.get_atom_masses(), [P.M_H])
+    for (step, rec), t in zip(sim.wfns, (100, 200)):
+        cds, dw = si.get_wfns(t)
+        assert step == t
+        np.testing.assert_array_equal(cds, rec['coords'])
+        np.testing.assert_array_equal(dw, rec['desc_wts'])
     re = pv.dmc_restart(pv.HarmonicOscillator1D(k=k), str(tmp_path), 'ho', additional_timesteps=50)
     re.propagate()
     assert re._vref_vs_tau.shape == (350,) and np.all(re._vref_vs_tau[300:] != 0.0)
