@@ -17,6 +17,7 @@ from .sim import DMC_Sim, dmc_restart
 from .surrogate import SurrogateModel, SurrogatePotential, pair_groups
 from ._native import DMCError, load as load_library
 from .analysis import SimInfo
+from .initial_conditions import HarmonicAnalysis, InitialConditioner, MolFiniteDifference
 from .hdf5 import save_h5
 
 __version__ = '0.1.0'

@@ -233,3 +233,19 @@ def test_is_fp32_isotopologue_tracks_fp64(atoms):
         sim.propagate()
         vref[prec] = sim._vref_vs_tau.copy()
     np.testing.assert_allclose(vref['fp32'], vref['fp64'], rtol=0, atol=2e-5)
+
+
+@pytest.mark.parametrize('which', ['water', 'cluster'])
+def test_harmonic_analysis_with_a_device_potential(which):
+    """The Hessian's stencils evaluated by the device potential in one launch give the
+    frequencies of the host NumPy potential (FP64 ulp differences, amplified by 1/dx^2)."""
+    import pyvibdmc_amd as pv
+    if which == 'water':
+        eq, atoms, m = P.water_eq(), ['H', 'H', 'O'], [P.M_H, P.M_H, P.M_O]
+        dev, host = pv.MorseBendTriatomic(), P.morse_bend
+    else:
+        eq, atoms, m = P.cluster6_eq(), ['O', 'H', 'H', 'O', 'H', 'H'], [P.M_O, P.M_H, P.M_H, P.M_O, P.M_H, P.M_H]
+        dev, host = pv.TwoMonomerCluster(), P.cluster6
+    fd, vd = pv.HarmonicAnalysis(eq, atoms, dev, masses=m).run()
+    fh, vh = pv.HarmonicAnalysis(eq, atoms, pv.Potential_Direct(host), masses=m).run()
+    np.testing.assert_allclose(fd[6:], fh[6:], rtol=0, atol=1e-3)
