@@ -15,6 +15,9 @@ Contents
                Adam), with bfloat16 rounding emulation.
 ``eigen``      Exact discrete-time-step ground-state energy of the reference's
                branching scheme for a 1-D harmonic oscillator (grid eigenvalue).
+``initial``    The reference's element-by-element finite-difference Hessian
+               (initial_conditioner/harmonic_analysis.py) and the Wilson GF-matrix
+               frequencies of a bent triatomic that pin it.
 ``cpu_ref``    The CPU baseline: the oracle driven like the reference's
                ``Potential`` manager (PES fanned out over a process pool).
 
