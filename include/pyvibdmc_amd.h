@@ -41,7 +41,8 @@ enum pvd_pes_kind {
   PVD_PES_HARMONIC_X = 1, /* V = p0 * x^2, x = first component of atom 0                */
   PVD_PES_MORSE_1D = 2,   /* V = p0 * (1 - exp(p1 * x))^2  (p1 = -alpha)                */
   PVD_PES_MORSE_BEND = 3, /* idx = (h1, h2, o); p = (De, a, re, kb, theta0)             */
-  PVD_PES_CLUSTER6 = 4    /* [O,H,H,O,H,H]; p = (De,a,re,kb,theta0, De_oo,a_oo,re_oo, qO,qH) */
+  PVD_PES_CLUSTER6 = 4,   /* [O,H,H,O,H,H]; p = (De,a,re,kb,theta0, De_oo,a_oo,re_oo, qO,qH) */
+  PVD_PES_PS_WATER = 5    /* Partridge-Schwenke water (h2opes_v2.f); idx = (h1, h2, o); no gradient */
 };
 enum pvd_trial_kind {
   PVD_TRIAL_NONE = 0,

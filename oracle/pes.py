@@ -349,3 +349,78 @@ def finite_diff(cds, trial_func, dx=0.001):
             first[:, a, d] = (tmp[:, 2] - tmp[:, 0]) / (2 * dx)
             sec[:, a, d] = ((tmp[:, 0] - 2.0 * tmp[:, 1] + tmp[:, 2]) / dx ** 2)
     return first, sec, tmp[:, 1]
+
+
+# --- Partridge-Schwenke water surface ------------------------------------------------------
+def _ps_table():
+    """The surface's coefficient table, read from the header the device kernels compile
+    (pyvibdmc_amd/csrc/pvd_ps_table.h, written by gen_ps_table.py from the reference's data)."""
+    import os
+    import re
+    path = os.path.join(os.path.dirname(os.path.dirname(os.path.abspath(__file__))), 'pyvibdmc_amd', 'csrc',
+                        'pvd_ps_table.h')
+    s = open(path).read()
+    par = {k: float(v) for k, v in re.findall(r'#define PVD_PS_([A-Z0-9]+) ([-0-9.e+]+)\n', s)}
+
+    def block(name):
+        m = re.search(r'#define ' + name + r' \\\n(.*?)(?:\n//|\Z)', s, re.S)
+        return [t for t in re.split(r'[,\s\\]+', m.group(1)) if t]
+    c = np.array([float(t) for t in block('PVD_PS_COEFFS')])
+    e = np.array([int(t) for t in block('PVD_PS_EXPONENTS')]).reshape(-1, 3)
+    return par, c, e
+
+
+_PS = None
+
+
+def ps_water(cds):
+    """Partridge-Schwenke water surface (hartree) of [H, H, O] walkers (bohr), in the
+    operation order of the reference's Fortran (calc_h2o_pot.f:12-27 for the internal
+    coordinates, h2opes_v2.f:460-489 for the surface): r1, r2 the O-H distances, theta from
+    acos(cos), x1 = (r1 - re)/re, x2 = (r2 - re)/re, x3 = cos(theta) - cos(theta_e);
+    V = [sum_j c_j (x1^a x2^b + x2^a x1^b) x3^k] exp(-b1 ((r1-re)^2 + (r2-re)^2)) + c_0
+        + V_OH(r1) + V_OH(r2) + V_HH(r_HH)."""
+    global _PS
+    if _PS is None:
+        _PS = _ps_table()
+    par, c, e = _PS
+    h1, h2, o = cds[:, 0], cds[:, 1], cds[:, 2]
+    d1 = o - h1
+    d2 = o - h2
+    r1s = ((0.0 + d1[:, 0] * d1[:, 0]) + d1[:, 1] * d1[:, 1]) + d1[:, 2] * d1[:, 2]
+    r2s = ((0.0 + d2[:, 0] * d2[:, 0]) + d2[:, 1] * d2[:, 1]) + d2[:, 2] * d2[:, 2]
+    ct = ((0.0 + d1[:, 0] * d2[:, 0]) + d1[:, 1] * d2[:, 1]) + d1[:, 2] * d2[:, 2]
+    r1 = np.sqrt(r1s)
+    r2 = np.sqrt(r2s)
+    th = np.arccos(ct / r1 / r2)
+    reoh = par['REOH']
+    x1 = (r1 - reoh) / reoh
+    x2 = (r2 - reoh) / reoh
+    cth = np.cos(th)
+    x3 = cth - par['CE']
+    rhh = np.sqrt((r1 * r1 + r2 * r2) - 2.0 * r1 * r2 * cth)
+    vhh = par['PHH1'] * np.exp(-par['PHH2'] * rhh)
+    ex = np.exp(-par['ALPHAOH'] * (r1 - par['ROH']))
+    voh1 = par['DEOH'] * ex * (ex - 2.0)
+    ex = np.exp(-par['ALPHAOH'] * (r2 - par['ROH']))
+    voh2 = par['DEOH'] * ex * (ex - 2.0)
+    f1 = [np.ones_like(x1)]
+    f2 = [np.ones_like(x1)]
+    f3 = [np.ones_like(x1)]
+    for _ in range(14):
+        f1.append(f1[-1] * x1)
+        f2.append(f2[-1] * x2)
+        f3.append(f3[-1] * x3)
+    v = np.zeros_like(x1)
+    for j in range(1, len(c)):
+        a, b, k = e[j]
+        v = v + c[j] * (f1[a] * f2[b] + f1[b] * f2[a]) * f3[k]
+    damp = np.exp(-par['B1'] * ((r1 - reoh) * (r1 - reoh) + (r2 - reoh) * (r2 - reoh)))
+    return (((v * damp + c[0]) + voh1) + voh2) + vhh
+
+
+def ps_eq():
+    """The surface's equilibrium: r_e = 0.958649 angstrom, theta_e = 104.3475 degrees, [H, H, O]."""
+    re_ = 0.958649 / 0.529177249
+    t = np.deg2rad(104.3475)
+    return np.array([[re_, 0.0, 0.0], [re_ * np.cos(t), re_ * np.sin(t), 0.0], [0.0, 0.0, 0.0]])

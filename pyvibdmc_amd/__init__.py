@@ -10,7 +10,8 @@ HarmonicTrial1D, GaussianProductTrial.  Surrogate potential: SurrogatePotential.
 """
 from .constants import Constants, get_atomic_num, get_atomic_string
 from .potentials import (Potential, Potential_NoMP, NN_Potential, Potential_Direct, DevicePotential,
-                         HarmonicOscillator1D, MorseOscillator1D, MorseBendTriatomic, TwoMonomerCluster)
+                         HarmonicOscillator1D, MorseOscillator1D, MorseBendTriatomic, TwoMonomerCluster,
+                         PartridgeSchwenkeWater)
 from .impsamp import (ImpSamp, ImpSampManager, ImpSampManager_NoMP, DeviceTrial, HarmonicTrial1D,
                       GaussianProductTrial, water_trial, water_trial_widths, cluster6_trial)
 from .sim import DMC_Sim, dmc_restart

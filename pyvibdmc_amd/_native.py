@@ -14,7 +14,7 @@ MAX_COMP = 96
 
 PVD_FP64, PVD_FP32 = 0, 1
 PVD_DISCRETE, PVD_CONTINUOUS = 0, 1
-PES_EXTERNAL, PES_HARMONIC_X, PES_MORSE_1D, PES_MORSE_BEND, PES_CLUSTER6 = 0, 1, 2, 3, 4
+PES_EXTERNAL, PES_HARMONIC_X, PES_MORSE_1D, PES_MORSE_BEND, PES_CLUSTER6, PES_PS_WATER = 0, 1, 2, 3, 4, 5
 TRIAL_NONE, TRIAL_HARMONIC_1D, TRIAL_GAUSS, TRIAL_HOST = 0, 1, 2, 3
 
 ST_POP, ST_WEIGHT, ST_CAPACITY, ST_BRANCH, ST_SPIN, E_BAD, E_HIP = 1, 3, 4, 5, 6, 7, 8

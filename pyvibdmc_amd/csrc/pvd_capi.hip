@@ -692,6 +692,7 @@ int pvd_pes_eval(const pvd_pes_desc* pes, int32_t precision, int32_t n_atoms, in
 int pvd_pes_vjp(const pvd_pes_desc* pes, int32_t precision, int32_t n_atoms, int32_t n_dim, const void* coords,
                 int64_t n, const void* grad_out, void* grad_coords, void* stream) {
   if (!pes || !coords || !grad_out || !grad_coords) return fail(E_BAD, "null argument");
+  if (pes->kind == PES_PS_WATER) return fail(E_BAD, "this potential has no analytic gradient");
   if (n <= 0) return 0;
   const int np = n_atoms * n_dim;
   return precision == PVD_FP64

@@ -6,7 +6,10 @@
 // (parity / FP64 runs).  Operation order in the FP64 forms follows the NumPy
 // expressions of the oracle so that FP64 results agree with it to a few ulps.
 #pragma once
+#include <utility>
+
 #include "pvd_common.hpp"
+#include "pvd_ps_table.h"
 
 namespace pvd {
 
@@ -16,6 +19,7 @@ enum PesKind : int {
   PES_MORSE_1D = 2,    // V = De*(t*t), t = 1 - exp(-alpha*x)
   PES_MORSE_BEND = 3,  // two Morse O-H stretches + harmonic bend, atoms (h1, h2, o)
   PES_CLUSTER6 = 4,    // two Morse+bend monomers + O-O Morse + inter-monomer Coulomb
+  PES_PS_WATER = 5,    // Partridge-Schwenke water surface, atoms (h1, h2, o)
 };
 
 struct PesParams {
@@ -163,6 +167,78 @@ PVD_DEV R cluster6_v(const PesParams& P, const R* x) {
   return v;
 }
 
+// ------------------------------------------------------------ Partridge-Schwenke water
+// The surface of H. Partridge and D. W. Schwenke (J. Chem. Phys. 106, 4618 (1997)) as the
+// reference evaluates it (sample_potentials/FortPots/Partridge_Schwenke_H2O/calc_h2o_pot.f,
+// h2opes_v2.f:460-489):
+//   V = [sum_j c_j (x1^a x2^b + x2^a x1^b) x3^k] exp(-b1 ((r1-re)^2 + (r2-re)^2)) + c_0
+//       + D ex (ex - 2) for each O-H distance (ex = exp(-alpha (r - r0))) + A exp(-b r_HH),
+//   x1 = (r1 - re)/re, x2 = (r2 - re)/re, x3 = cos(theta) - cos(theta_e).
+// The 245-term table (pvd_ps_table.h) is unrolled at compile time, so the powers of x1, x2,
+// x3 stay in registers and each coefficient is an immediate.  Double precision keeps the
+// reference's order of operations (terms summed in table order); single precision fuses
+// each term into one multiply-add.
+inline constexpr double PS_C[PVD_PS_NTERMS] = {PVD_PS_COEFFS};
+inline constexpr unsigned char PS_E[3 * PVD_PS_NTERMS] = {PVD_PS_EXPONENTS};
+
+template <typename R, int J>
+PVD_DEV R ps_term(const R* f1, const R* f2, const R* f3, R v) {
+  constexpr int a = PS_E[3 * J], b = PS_E[3 * J + 1], k = PS_E[3 * J + 2];
+  constexpr double c = PS_C[J];
+  if constexpr (sizeof(R) == 8) {
+    return v + (R(c) * (f1[a] * f2[b] + f1[b] * f2[a])) * f3[k];
+  } else {
+    return __builtin_fmaf(R(c) * (f1[a] * f2[b] + f1[b] * f2[a]), f3[k], v);
+  }
+}
+
+template <typename R, int... J>
+PVD_DEV R ps_poly(const R* f1, const R* f2, const R* f3, std::integer_sequence<int, J...>) {
+  R v = R(0);
+  ((v = ps_term<R, J + 1>(f1, f2, f3, v)), ...);  // terms 1 .. 244 in table order
+  return v;
+}
+
+// x: [h1, h2, o] components 0..8
+template <typename R>
+PVD_DEV R ps_water_v(const R* x) {
+  R r1s = R(0), r2s = R(0), ct = R(0);
+#pragma unroll
+  for (int c = 0; c < 3; ++c) {
+    const R d1 = x[6 + c] - x[c], d2 = x[6 + c] - x[3 + c];
+    r1s = r1s + d1 * d1;
+    r2s = r2s + d2 * d2;
+    ct = ct + d1 * d2;
+  }
+  const R r1 = pvd_sqrt<R>(r1s), r2 = pvd_sqrt<R>(r2s);
+  const R re = R(PVD_PS_REOH);
+  R cth;
+  if constexpr (sizeof(R) == 8) {
+    cth = cos(acos((ct / r1) / r2));  // the reference's theta = acos(...), then cos(theta)
+  } else {
+    cth = ct / (r1 * r2);
+  }
+  const R x1 = (r1 - re) / re, x2 = (r2 - re) / re, x3 = cth - R(PVD_PS_CE);
+  const R rhh = pvd_sqrt<R>((r1 * r1 + r2 * r2) - ((R(2) * r1) * r2) * cth);
+  const R vhh = R(PVD_PS_PHH1) * pvd_exp<R>(-(R(PVD_PS_PHH2) * rhh));
+  R ex = pvd_exp<R>(-(R(PVD_PS_ALPHAOH) * (r1 - R(PVD_PS_ROH))));
+  const R voh1 = (R(PVD_PS_DEOH) * ex) * (ex - R(2));
+  ex = pvd_exp<R>(-(R(PVD_PS_ALPHAOH) * (r2 - R(PVD_PS_ROH))));
+  const R voh2 = (R(PVD_PS_DEOH) * ex) * (ex - R(2));
+  R f1[15], f2[15], f3[15];
+  f1[0] = f2[0] = f3[0] = R(1);
+#pragma unroll
+  for (int j = 1; j < 15; ++j) {
+    f1[j] = f1[j - 1] * x1;
+    f2[j] = f2[j - 1] * x2;
+    f3[j] = f3[j - 1] * x3;
+  }
+  const R poly = ps_poly<R>(f1, f2, f3, std::make_integer_sequence<int, PVD_PS_NTERMS - 1>());
+  const R d1 = r1 - re, d2 = r2 - re;
+  const R damp = pvd_exp<R>(-(R(PVD_PS_B1) * (d1 * d1 + d2 * d2)));
+  return (((poly * damp + R(PS_C[0])) + voh1) + voh2) + vhh;
+}
+
 template <typename R, int NP>
 PVD_DEV R pes_eval(const PesParams& P, const R* x) {
   switch (P.kind) {
@@ -186,6 +262,14 @@ PVD_DEV R pes_eval(const PesParams& P, const R* x) {
       break;
     case PES_CLUSTER6:
       if constexpr (NP == 18) return cluster6_v<R>(P, x);
+      break;
+    case PES_PS_WATER:
+      if constexpr (NP == 9) {
+        if (P.idx[0] == 0 && P.idx[1] == 1 && P.idx[2] == 2) return ps_water_v<R>(x);
+        R g[9];
+        gather3<R, NP>(x, P.idx, g);
+        return ps_water_v<R>(g);
+      }
       break;
     default:
       break;

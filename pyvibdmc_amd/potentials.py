@@ -344,3 +344,24 @@ class TwoMonomerCluster(DevicePotential):
         self.desc.kind = N.PES_CLUSTER6
         for i, v in enumerate((De, a, re, kb, theta0, De_oo, a_oo, re_oo, q_o, q_h)):
             self.desc.p[i] = float(v)
+
+
+class PartridgeSchwenkeWater(DevicePotential):
+    """The Partridge-Schwenke water surface (J. Chem. Phys. 106, 4618 (1997)), the reference's
+    sample Fortran potential (sample_potentials/FortPots/Partridge_Schwenke_H2O/h2o_potential.py
+    ``water_pot``) as a device potential.  ``atoms`` gives the walker indices of (H1, H2, O);
+    the default [H, H, O] order is the reference's.  Energies only (no analytic gradient)."""
+    n_atoms = 3
+
+    def __init__(self, atoms=(0, 1, 2)):
+        super().__init__()
+        self.desc.kind = N.PES_PS_WATER
+        for i, v in enumerate(atoms):
+            self.desc.idx[i] = int(v)
+
+    @staticmethod
+    def equilibrium():
+        """[H, H, O] at r_e = 0.958649 angstrom, theta_e = 104.3475 degrees (bohr)."""
+        re_ = 0.958649 / 0.529177249
+        t = np.deg2rad(104.3475)
+        return np.array([[re_, 0.0, 0.0], [re_ * np.cos(t), re_ * np.sin(t), 0.0], [0.0, 0.0, 0.0]])

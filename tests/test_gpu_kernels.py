@@ -25,10 +25,12 @@ def cases():
         ('ho_x', pv.HarmonicOscillator1D(k=k), lambda c: P.harmonic_x(c, k), None, x1),
         ('morse_bend', pv.MorseBendTriatomic(), P.morse_bend, P.morse_bend_grad, water(2000, 2)),
         ('cluster6', pv.TwoMonomerCluster(), P.cluster6, P.cluster6_grad, cluster(2000, 3)),
+        ('ps_water', pv.PartridgeSchwenkeWater(), P.ps_water, None,
+         P.ps_eq()[None] + np.random.RandomState(4).normal(0, 0.1, (2000, 3, 3))),
     ]
 
 
-@pytest.mark.parametrize('i', range(3))
+@pytest.mark.parametrize('i', range(4))
 def test_device_pes_matches_oracle_fp64_and_fp32(i):
     import torch
     name, dev, ref, _, x = cases()[i]
@@ -249,3 +251,40 @@ def test_harmonic_analysis_with_a_device_potential(which):
     fd, vd = pv.HarmonicAnalysis(eq, atoms, dev, masses=m).run()
     fh, vh = pv.HarmonicAnalysis(eq, atoms, pv.Potential_Direct(host), masses=m).run()
     np.testing.assert_allclose(fd[6:], fh[6:], rtol=0, atol=1e-3)
+
+
+
+def test_ps_water_matches_the_reference_fortran():
+    """The device Partridge-Schwenke surface against energies the reference's own Fortran
+    wrote (tests/golden/ps_water.npz): FP64 to 1e-12 relative; FP32 to 2e-6 hartree (0.4 cm^-1)
+    up to 0.5 hartree above the minimum.  Walker order [O, H, H] through the atom indices
+    gives the same energies."""
+    import torch
+    import pyvibdmc_amd as pv
+    from pyvibdmc_amd import _native as N
+    from tests.golden_cases import GOLDEN
+    g = np.load(GOLDEN + '/ps_water.npz')
+    x, v = g['cds'], g['v']
+    dev = pv.PartridgeSchwenkeWater()
+    np.testing.assert_allclose(dev.getpot(x), v, rtol=1e-12, atol=1e-15)
+    np.testing.assert_allclose(pv.PartridgeSchwenkeWater(atoms=(1, 2, 0)).getpot(x[:, [2, 0, 1]]), v,
+                               rtol=1e-12, atol=1e-15)
+    v32 = dev.eval_device(torch.from_numpy(x.astype(np.float32)).cuda(), precision=N.PVD_FP32).cpu().numpy()
+    ref32 = P.ps_water(x.astype(np.float32).astype(np.float64))
+    low = ref32 < 0.5
+    np.testing.assert_allclose(v32[low], ref32[low], rtol=0, atol=2e-6)
+
+
+def test_ps_water_dmc_zpe_matches_the_reference_runs():
+    """Plain discrete DMC on the Partridge-Schwenke surface, dt = 5 like the reference's
+    tutorial runs (pyvibdmc/sample_sim_data: five runs average 4634.1 cm^-1 with a spread of
+    2.2 cm^-1): 60k walkers x 4000 steps FP32 give the same ZPE within the combined error."""
+    import pyvibdmc_amd as pv
+    n = 60_000
+    start = P.ps_eq()[None] + np.random.RandomState(5).normal(0, 0.05, (n, 3, 3))
+    sim = pv.DMC_Sim(num_walkers=n, num_timesteps=4000, atoms=['H', 'H', 'O'], delta_t=5,
+                     potential=pv.PartridgeSchwenkeWater(), start_structures=start, precision='fp32', seed=6,
+                     write_files=False, equil_steps=10 ** 9, wfn_every=10 ** 9, chkpt_every=10 ** 9)
+    sim.propagate()
+    zpe = sim.zpe(onwards=1000)
+    assert abs(zpe - 4634.1) < 4.0, zpe
