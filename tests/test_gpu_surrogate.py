@@ -1,9 +1,12 @@
 """GPU surrogate kernels against the CPU oracle (oracle/mlp.py).
 
-Bars: descriptors to float32 rounding (1e-6 relative); forward energies and parameter
-gradients to the BF16 operand precision (2e-2 of the tensor norm: both sides round the same
-operands to bfloat16, and accumulation order flips a few roundings); training reduces the
-loss and reaches sub-milli-hartree accuracy on the triatomic surface.
+Bars: descriptors to float32 rounding (1e-6 relative).  Forward energies to 2e-3 of the
+energy spread, parameter gradients to 5e-4 of each tensor's norm and the loss to 1e-5: both
+sides round the same operands to bfloat16, so only the accumulation order separates them (it
+flips the rounding of a few activations).  Measured on the GPU: forward <= 3.6e-4, gradients
+<= 5.3e-5, loss <= 1e-7 over three seeds each of the 3 -> 128, 3 -> 256 and 15 -> 256
+networks.  Training reduces the loss and reaches sub-milli-hartree accuracy on the triatomic
+surface.
 """
 import numpy as np
 import pytest
@@ -16,6 +19,17 @@ pytestmark = pytest.mark.gpu
 
 def water_walkers(n, seed, noise=0.08):
     return P.water_eq()[None] + np.random.RandomState(seed).normal(0, noise, (n, 3, 3))
+
+
+WATER = ['H', 'H', 'O']
+CLUSTER = ['O', 'H', 'H', 'O', 'H', 'H']
+NETS = [(WATER, 128), (WATER, 256), (CLUSTER, 256)]  # triatomic 3 -> 128, its wide form, hexamer 15 -> 256
+
+
+def walkers_for(elements, n, seed):
+    if len(elements) == 3:
+        return water_walkers(n, seed), P.morse_bend
+    return P.cluster6_eq()[None] + np.random.RandomState(seed).normal(0, 0.08, (n, 6, 3)), P.cluster6
 
 
 def oracle_net(model):
@@ -46,17 +60,17 @@ def test_descriptors_match_oracle(elements, shape):
     np.testing.assert_allclose(got, O.descriptor(x, pairs, sizes), rtol=2e-6)
 
 
-@pytest.mark.parametrize('hidden', [128, 256])
-def test_forward_matches_oracle(hidden):
+@pytest.mark.parametrize('elements,hidden', NETS)
+def test_forward_matches_oracle(elements, hidden):
     import pyvibdmc_amd as pv
-    m = pv.SurrogateModel(['H', 'H', 'O'], hidden=hidden, seed=3)
-    x = water_walkers(3000, 4)
-    m.set_normalisation(x, P.morse_bend(x))
+    m = pv.SurrogateModel(elements, hidden=hidden, seed=3)
+    x, pes = walkers_for(elements, 3000, 4)
+    m.set_normalisation(x, pes(x))
     v = m.energy(x)
     y, _ = oracle_net(m).forward(normalised_features(m, x))
     ref = y * m.desc.ystd + m.desc.ymean
     err = np.linalg.norm(v - ref) / np.linalg.norm(ref - ref.mean())
-    assert err < 2e-2, err
+    assert err < 2e-3, err
 
 
 @pytest.mark.parametrize('hidden', [128, 256])
@@ -79,18 +93,18 @@ def test_forward_with_quadratic_baseline_matches_oracle(hidden):
     v = m.energy(x)
     ref = y * m.desc.ystd + m.desc.ymean + base
     err = np.linalg.norm(v - ref) / np.linalg.norm(ref - ref.mean())
-    assert err < 2e-2, err
+    assert err < 2e-3, err
     m.desc.ystd = 0.0
     np.testing.assert_allclose(m.energy(x), m.desc.ymean + base, rtol=0, atol=1e-6)
 
 
-@pytest.mark.parametrize('hidden', [128, 256])
-def test_gradients_match_oracle(hidden):
+@pytest.mark.parametrize('elements,hidden', NETS)
+def test_gradients_match_oracle(elements, hidden):
     import torch
     import pyvibdmc_amd as pv
-    m = pv.SurrogateModel(['H', 'H', 'O'], hidden=hidden, seed=5)
-    x = water_walkers(2048, 6)
-    e = P.morse_bend(x)
+    m = pv.SurrogateModel(elements, hidden=hidden, seed=5)
+    x, pes = walkers_for(elements, 2048, 6)
+    e = pes(x)
     m.set_normalisation(x, e)
     xt = m.features(x, normalise=True)
     t = torch.from_numpy(((e - m.desc.ymean) / m.desc.ystd).astype(np.float32)).cuda()
@@ -98,7 +112,7 @@ def test_gradients_match_oracle(hidden):
     gd = torch.clone(m.grads).cpu().numpy()
     net = oracle_net(m)
     lo, yo, go = net.loss_grads(xt.cpu().numpy().T.copy(), t.cpu().numpy().astype(np.float64))
-    assert abs(float(loss) - lo) <= 2e-2 * lo
+    assert abs(float(loss) - lo) <= 1e-5 * lo
     H, D = m.H, m.D
     o = 0
     for k, sz in (('W0', H * D), ('b0', H), ('W1', H * H), ('b1', H), ('W2', H * H), ('b2', H), ('w3', H), ('b3', 1)):
@@ -106,7 +120,7 @@ def test_gradients_match_oracle(hidden):
         b = go[k].reshape(-1)
         o += sz
         rel = np.linalg.norm(a - b) / max(np.linalg.norm(b), 1e-30)
-        assert rel < 3e-2, (k, rel)
+        assert rel < 5e-4, (k, rel)
 
 
 def test_training_reduces_loss_and_learns_the_surface():
@@ -121,22 +135,23 @@ def test_training_reduces_loss_and_learns_the_surface():
     assert rms < 5e-4, rms  # hartree
 
 
-def test_adam_matches_oracle_step():
-    import pyvibdmc_amd as pv
-    m = pv.SurrogateModel(['H', 'H', 'O'], hidden=128, seed=11)
-    p0 = m.param_dict()
-    g = np.random.RandomState(0).normal(0, 1, m.params.numel()).astype(np.float32)
+def test_adam_matches_oracle_ten_steps():
+    """The device Adam against the oracle's (itself pinned to torch.optim.Adam) over ten
+    steps of the 15 -> 256 network's parameters, with gradients spanning four decades."""
     import torch
-    m.grads.copy_(torch.from_numpy(g).cuda())
-    m.adam_step(lr=1e-3)
-    m.grads.copy_(torch.from_numpy(0.5 * g).cuda())
-    m.adam_step(lr=1e-3)
+    import pyvibdmc_amd as pv
+    m = pv.SurrogateModel(CLUSTER, hidden=256, seed=11)
+    p0 = m.param_dict()
     flat0 = np.concatenate([p0[k].ravel() for k in ('W0', 'b0', 'W1', 'b1', 'W2', 'b2', 'w3', 'b3')])
     pp = {'x': flat0.copy()}
     opt = O.Adam(pp, lr=1e-3)
-    opt.step(pp, {'x': g})
-    opt.step(pp, {'x': 0.5 * g})
-    np.testing.assert_allclose(m.params.cpu().numpy(), pp['x'], rtol=1e-5, atol=1e-7)
+    rs = np.random.RandomState(0)
+    for _ in range(10):
+        g = (rs.normal(0, 1, m.params.numel()) * 10.0 ** rs.uniform(-3, 1)).astype(np.float32)
+        m.grads.copy_(torch.from_numpy(g).cuda())
+        m.adam_step(lr=1e-3)
+        opt.step(pp, {'x': g})
+    np.testing.assert_allclose(m.params.cpu().numpy(), pp['x'], rtol=2e-5, atol=2e-7)
 
 
 def test_surrogate_dmc_reproduces_analytic_zpe_and_beats_its_baseline():
@@ -205,4 +220,4 @@ def test_persistent_inference_over_many_chunks(hidden):
     y, _ = oracle_net(m).forward(normalised_features(m, xs))
     ref = y * m.desc.ystd + m.desc.ymean + O.quad_terms(normalised_features(m, xs)) @ np.array(m.desc.quad[:10])
     err = np.linalg.norm(v_one - ref) / np.linalg.norm(ref - ref.mean())
-    assert err < 2e-2, err
+    assert err < 2e-3, err
