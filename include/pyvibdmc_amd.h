@@ -211,8 +211,17 @@ int pvd_trial_vjp(const pvd_trial_desc* trial, int32_t precision, int32_t n_atom
  * order W0 (H x D), b0, W1 (H x H), b1, W2, b2, w3 (H), b3. */
 typedef struct pvd_mlp {
   int32_t D, H;
-  int32_t npairs;          /* descriptor pairs, D <= npairs <= 15 */
-  int32_t pair_i[15], pair_j[15], pair_group[15];
+  /* Descriptor (simulation_utilities/tensorflow_descriptors/distance_descriptors.py:6-214,
+   * DistIt with method 'coulomb'): the inverse distances of all atom pairs after permuting like
+   * atoms, position pairs (0,1), (0,2), ..., (1,2), ...  n_atoms = 0: a network trained on
+   * given features only; otherwise 2..6 atoms and D = n_atoms (n_atoms - 1) / 2. */
+  int32_t n_atoms;
+  int32_t z[6];                /* nuclear charges; the sort key is the Coulomb-matrix row norm */
+  int32_t sorted_atoms[6];     /* DistIt sorted_atoms: sublists concatenated, covering every atom */
+  int32_t sorted_atoms_len[6]; /* their lengths; n_sorted_atoms = 0: no atom sorting */
+  int32_t n_sorted_atoms;
+  int32_t sorted_groups[6];    /* DistIt sorted_groups: n_groups x group_size positions */
+  int32_t n_groups, group_size;  /* n_groups < 2: no group sorting */
   int32_t use_range;       /* 1: walkers outside [lo, hi] use the fallback potential */
   float xmean[32], xinv[32], lo[32], hi[32];
   float ymean, ystd;

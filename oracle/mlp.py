@@ -6,11 +6,15 @@ operands to bfloat16 (round to nearest even) and accumulates in float64, so the 
 up to accumulation order and the rare bfloat16 rounding flip that order causes.
 
 Descriptor (reference ``simulation_utilities/tensorflow_descriptors/distance_descriptors.py``,
-``atm_atm_dists`` :102-113 and the like-atom sorting of :115-152): inverse interatomic
-distances over a list of atom pairs, sorted in descending order within each group of pairs
-of the same element pair.  Pinned by permutation invariance (swapping like atoms leaves the
-descriptor unchanged, the property the reference tests at tests/test_cupy_dist.py:107,142)
-and by closed-form distances.
+``DistIt`` with method 'coulomb': ``atm_atm_dists`` :102-113, ``sort_atoms`` :115-131,
+``sort_groups`` :133-152, ``run`` :177-214): like atoms are permuted by descending
+Coulomb-matrix row norm within each ``sorted_atoms`` sublist, then equal-size position
+groups (``sorted_groups``) by descending summed norm; the features are the inverse distances
+of the permuted atoms for position pairs (0,1), (0,2), ..., (1,2), ... -- the reference's
+Coulomb values divided by z_i z_j.  Pinned by vectors the reference's DistIt wrote
+(``tests/golden/distit.npz``), by the exact permutation invariance its own tests assert
+(pyvibdmc/tests/test_cupy_dist.py:80-107, :110-142, on their fixtures) and by closed-form
+distances.
 
 Network: x_n = (desc - mean) * inv_std; h0 = silu(W0 x_n + b0); h1 = silu(W1 h0 + b1);
 h2 = silu(W2 h1 + b2); y = w3 . h2 + b3; V = y * y_std + y_mean + q(x_n), q a quadratic
@@ -42,33 +46,69 @@ def dsilu(z):
 
 
 # ---- descriptors ----------------------------------------------------------------------
-def pair_groups(elements):
-    """All atom pairs (i < j) grouped by unordered element pair, group order of first
-    appearance; returns (pairs, group_sizes) with pairs listed group by group."""
-    pairs = [(i, j) for i in range(len(elements)) for j in range(i + 1, len(elements))]
-    keys = []
-    for i, j in pairs:
-        k = tuple(sorted((elements[i], elements[j])))
-        if k not in keys:
-            keys.append(k)
-    out, sizes = [], []
-    for k in keys:
-        g = [(i, j) for (i, j) in pairs if tuple(sorted((elements[i], elements[j]))) == k]
-        out += g
-        sizes.append(len(g))
-    return out, sizes
-
-
-def descriptor(cds, pairs, sizes):
-    """Inverse distances of `pairs`, sorted descending within each group."""
+def inverse_distances(cds):
+    """1/r of every atom pair (i < j) in row order (distance_descriptors.py:102-113), (N, P)."""
     cds = np.asarray(cds, dtype=np.float64)
-    inv = np.stack([1.0 / np.linalg.norm(cds[:, i] - cds[:, j], axis=1) for i, j in pairs], axis=1)
-    out = np.empty_like(inv)
-    o = 0
-    for s in sizes:
-        out[:, o:o + s] = -np.sort(-inv[:, o:o + s], axis=1)
-        o += s
-    return out
+    m = cds.shape[1]
+    return np.stack([1.0 / np.linalg.norm(cds[:, i] - cds[:, j], axis=1)
+                     for i in range(m) for j in range(i + 1, m)], axis=1)
+
+
+def coulomb_row_norms(cds, zs):
+    """Row norms of the Coulomb matrix z_i z_j / r_ij with diagonal 0.5 z_i^2.4
+    (distance_descriptors.py:154-166), (N, M)."""
+    cds = np.asarray(cds, dtype=np.float64)
+    zs = np.asarray(zs, dtype=np.float64)
+    m = cds.shape[1]
+    c = np.zeros((cds.shape[0], m, m))
+    for i in range(m):
+        c[:, i, i] = 0.5 * zs[i] ** 2.4
+        for j in range(m):
+            if j != i:
+                c[:, i, j] = zs[i] * zs[j] / np.linalg.norm(cds[:, i] - cds[:, j], axis=1)
+    return np.sqrt(np.sum(c * c, axis=2))
+
+
+def atom_order(norms, sorted_atoms=None, sorted_groups=None):
+    """Per geometry, the atom placed at each position: like atoms by descending norm inside
+    each sublist (position sublist[r] gets the r-th largest, ties to the lower index), then
+    position groups by descending summed norm (ties to the lower group), (N, M) int."""
+    n, m = norms.shape
+    perm = np.tile(np.arange(m), (n, 1))
+    if sorted_atoms is not None:
+        for sub in sorted_atoms:
+            sub = list(sub)
+            if len(sub) < 2:
+                continue
+            # stable sort by -norm: equal norms keep ascending atom order
+            order = np.argsort(-norms[:, sub], axis=1, kind='stable')
+            perm[:, sub] = np.asarray(sub)[order]
+    if sorted_groups is not None and len(sorted_groups) > 1:
+        g = np.asarray(sorted_groups)
+        pn = np.take_along_axis(norms, perm, axis=1)          # norm of the atom at each position
+        tot = np.stack([pn[:, grp].sum(axis=1) for grp in g], axis=1)
+        rank_order = np.argsort(-tot, axis=1, kind='stable')  # k-th largest group
+        new = perm.copy()
+        for k in range(len(g)):
+            src = g[rank_order[:, k]]                          # (N, group size) positions
+            new[:, g[k]] = np.take_along_axis(perm, src, axis=1)
+        perm = new
+    return perm
+
+
+def descriptor(cds, zs, sorted_atoms=None, sorted_groups=None):
+    """Inverse distances of the permuted atoms, position pairs in row order, (N, P)."""
+    cds = np.asarray(cds, dtype=np.float64)
+    perm = atom_order(coulomb_row_norms(cds, zs), sorted_atoms, sorted_groups)
+    return inverse_distances(np.take_along_axis(cds, perm[:, :, None], axis=1))
+
+
+def like_atoms(elements):
+    """One sublist per element, in order of first appearance."""
+    return [[i for i, x in enumerate(elements) if x == e] for e in dict.fromkeys(elements)]
+
+
+ZS = {'H': 1, 'D': 1, 'O': 8}
 
 
 # ---- quadratic baseline ---------------------------------------------------------------

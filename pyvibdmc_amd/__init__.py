@@ -15,7 +15,7 @@ from .potentials import (Potential, Potential_NoMP, NN_Potential, Potential_Dire
 from .impsamp import (ImpSamp, ImpSampManager, ImpSampManager_NoMP, DeviceTrial, HarmonicTrial1D,
                       GaussianProductTrial, water_trial, water_trial_widths, cluster6_trial)
 from .sim import DMC_Sim, dmc_restart
-from .surrogate import SurrogateModel, SurrogatePotential, pair_groups
+from .surrogate import SurrogateModel, SurrogatePotential, like_atoms
 from ._native import DMCError, load as load_library
 from .analysis import SimInfo
 from .initial_conditions import HarmonicAnalysis, InitialConditioner, MolFiniteDifference

@@ -557,6 +557,7 @@ int pvd_sim_set_surrogate(pvd_sim* s, const pvd_mlp* net, const pvd_pes_desc* fa
   if (s->cfg.pes.kind != PES_EXTERNAL) return fail(E_BAD, "the simulation has a built-in device potential");
   if (!fallback) return fail(E_BAD, "the surrogate needs an analytic fallback potential");
   if (s->d != 3 || (s->M != 3 && s->M != 6)) return fail(E_BAD, "the surrogate supports 3 or 6 atoms in 3-D");
+  if (net->n_atoms != s->M) return fail(E_BAD, "the network's descriptor is for another number of atoms");
   if (judge_every < 0 || (judge_every > 0 && judge_stride < 1)) return fail(E_BAD, "bad judge schedule");
   if (!s->vsur) HIPCHK(hipMalloc((void**)&s->vsur, sizeof(double) * s->cap));
   if (judge_every > 0 && !s->jlog) {

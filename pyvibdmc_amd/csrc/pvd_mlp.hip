@@ -1,4 +1,4 @@
-// Surrogate potential: sorted inverse-distance descriptor + MLP (D -> H -> H -> H -> 1,
+// Surrogate potential: permutation-sorted inverse-distance descriptor + MLP (D -> H -> H -> H -> 1,
 // SiLU) on BF16 MFMA with FP32 accumulation, for inference inside the DMC step and for
 // training (forward with stored pre-activations, backward, weight gradients, Adam).
 //
@@ -14,6 +14,7 @@
 #include <mutex>
 
 #include <algorithm>
+#include <cmath>
 #include <cstring>
 #include <string>
 
@@ -26,6 +27,7 @@ typedef __attribute__((ext_vector_type(8))) __bf16 bf16x8;
 typedef __attribute__((ext_vector_type(4))) float f32x4;
 
 constexpr int MLP_MAXD = 32;
+constexpr int MLP_MAXA = 6;
 constexpr int MLP_MAXP = 15;
 constexpr int MLP_NQ = 1 + MLP_MAXP + MLP_MAXP * (MLP_MAXP + 1) / 2;  // 136
 
@@ -36,9 +38,13 @@ struct MlpNet {
   const float* b[3];
   const float* w3;
   const float* b3;
-  int npairs, use_range;
-  int pi[MLP_MAXP], pj[MLP_MAXP], grp[MLP_MAXP];
-  int cidx[MLP_MAXP];  // canonical pair index of descriptor slot k
+  int na, use_range;
+  float zz[MLP_MAXP];             // z_i z_j of atom pair q in order (0,1), (0,2), ..., (1,2), ...
+  float cd2[MLP_MAXA];            // squared Coulomb diagonal (0.5 z_a^2.4)^2
+  int cmask[MLP_MAXA];            // bit b: atom b is sorted together with atom a (0: a stays)
+  int slot[MLP_MAXA][MLP_MAXA];   // position of the r-th largest-norm member of a's sublist
+  int ng, gs;                     // ng > 1: ng position groups of gs positions are sorted
+  int gpos[MLP_MAXA][MLP_MAXA];   // position m of group g
   float xmean[MLP_MAXD], xinv[MLP_MAXD], lo[MLP_MAXD], hi[MLP_MAXD];
   float ymean, ystd;
   float quad[MLP_NQ];  // quadratic baseline on the normalised features
@@ -73,15 +79,28 @@ PVD_DEV float dsilu_f(float z) {
 // unit index of element j of input chunk c for lane group g (the permuted K order)
 PVD_DEV int phys_unit(int c, int g, int j) { return 16 * (2 * c + (j >> 2)) + 4 * g + (j & 3); }
 
-// descriptor of one walker: inverse distances of the pairs, sorted descending inside each
-// group by odd-even transposition (comparisons never cross a group boundary).  Distances
-// are computed for all NA(NA-1)/2 pairs in canonical order (0,1), (0,2), ..., (1,2), ...
-// with compile-time coordinate indices; M.cidx maps descriptor slot k to its canonical pair.
+// descriptor of one walker (the DistIt permutation sorting of
+// simulation_utilities/tensorflow_descriptors/distance_descriptors.py:115-152): atoms are
+// permuted, never pair distances, so the features stay the upper triangle of one inverse
+// distance matrix.
+//  1. sort key: the row norm of the Coulomb matrix (z_i z_j / r_ij, diagonal 0.5 z_i^2.4);
+//  2. sorted atoms: within each sublist, position sublist[r] receives the member with the
+//     r-th largest norm (ties: lower atom index first);
+//  3. sorted groups: equal-size position groups are reordered by the sum of their atoms'
+//     norms, largest first (ties: lower group index first);
+//  4. features: 1/r of the permuted atoms for position pairs (0,1), (0,2), ..., (1,2), ...
+//     (the Coulomb values divided by z_i z_j, a per-feature constant the normalisation
+//     removes).
+// The permutation is packed as 4-bit fields (position p holds atom (perm >> 4p) & 15), so
+// no lane indexes a register array at run time; every table lookup has a compile-time index.
 template <typename R, int NP>
 PVD_DEV void mlp_descriptor(const MlpNet& M, const R* x, float* f) {
   constexpr int NA = NP / 3, NPR = NA * (NA - 1) / 2;
-  static_assert(NPR <= MLP_MAXP, "too many atom pairs");
+  static_assert(NA <= MLP_MAXA && NPR <= MLP_MAXP, "too many atoms");
   float dc[NPR];
+  float rn2[NA];
+#pragma unroll
+  for (int a = 0; a < NA; ++a) rn2[a] = M.cd2[a];
   {
     int q = 0;
 #pragma unroll
@@ -91,30 +110,88 @@ PVD_DEV void mlp_descriptor(const MlpNet& M, const R* x, float* f) {
         const float d0 = (float)(x[3 * i] - x[3 * j]);
         const float d1 = (float)(x[3 * i + 1] - x[3 * j + 1]);
         const float d2 = (float)(x[3 * i + 2] - x[3 * j + 2]);
-        dc[q++] = 1.0f / __fsqrt_rn(d0 * d0 + d1 * d1 + d2 * d2);
+        dc[q] = 1.0f / __fsqrt_rn(d0 * d0 + d1 * d1 + d2 * d2);
+        const float c = M.zz[q] * dc[q];
+        rn2[i] += c * c;
+        rn2[j] += c * c;
+        ++q;
+      }
+  }
+  unsigned perm = 0;
+#pragma unroll
+  for (int a = 0; a < NA; ++a) {
+    int r = 0;
+#pragma unroll
+    for (int b = 0; b < NA; ++b)
+      if (b != a && ((M.cmask[a] >> b) & 1)) r += (rn2[b] > rn2[a] || (rn2[b] == rn2[a] && b < a)) ? 1 : 0;
+    int p = a;
+    if (M.cmask[a]) {
+#pragma unroll
+      for (int t = 0; t < NA; ++t)
+        if (r == t) p = M.slot[a][t];
+    }
+    perm |= (unsigned)a << (4 * p);
+  }
+  if (M.ng > 1) {
+    float tot[NA];
+#pragma unroll
+    for (int g = 0; g < NA; ++g) {
+      tot[g] = 0.0f;
+      if (g < M.ng) {
+#pragma unroll
+        for (int m = 0; m < NA; ++m) {
+          if (m < M.gs) {
+            const unsigned at = (perm >> (4 * M.gpos[g][m])) & 15u;
+            float nrm = 0.0f;
+#pragma unroll
+            for (int a = 0; a < NA; ++a)
+              if (at == (unsigned)a) nrm = __fsqrt_rn(rn2[a]);
+            tot[g] += nrm;
+          }
+        }
+      }
+    }
+    unsigned fp = perm;
+#pragma unroll
+    for (int g = 0; g < NA; ++g) {
+      if (g < M.ng) {
+        int k = 0;
+#pragma unroll
+        for (int h = 0; h < NA; ++h)
+          if (h < M.ng && h != g) k += (tot[h] > tot[g] || (tot[h] == tot[g] && h < g)) ? 1 : 0;
+#pragma unroll
+        for (int m = 0; m < NA; ++m) {
+          if (m < M.gs) {
+            int dst = 0;
+#pragma unroll
+            for (int kk = 0; kk < NA; ++kk)
+              if (kk == k) dst = M.gpos[kk][m];
+            const unsigned src = (perm >> (4 * M.gpos[g][m])) & 15u;
+            fp = (fp & ~(15u << (4 * dst))) | (src << (4 * dst));
+          }
+        }
+      }
+    }
+    perm = fp;
+  }
+  {
+    int q = 0;
+#pragma unroll
+    for (int p1 = 0; p1 < NA; ++p1)
+#pragma unroll
+      for (int p2 = p1 + 1; p2 < NA; ++p2) {
+        const int a1 = (int)((perm >> (4 * p1)) & 15u), a2 = (int)((perm >> (4 * p2)) & 15u);
+        const int lo = a1 < a2 ? a1 : a2, hi = a1 < a2 ? a2 : a1;
+        const int cq = lo * NA - lo * (lo + 1) / 2 + (hi - lo - 1);
+        float v = 0.0f;
+#pragma unroll
+        for (int s = 0; s < NPR; ++s)
+          if (cq == s) v = dc[s];
+        f[q++] = v;
       }
   }
 #pragma unroll
-  for (int k = 0; k < MLP_MAXP; ++k) {
-    float v = 0.0f;
-    if (k < NPR) {
-#pragma unroll
-      for (int q = 0; q < NPR; ++q)
-        if (M.cidx[k] == q) v = dc[q];
-    }
-    f[k] = v;
-  }
-#pragma unroll
-  for (int r = 0; r < NPR; ++r) {
-#pragma unroll
-    for (int k = (r & 1); k + 1 < NPR; k += 2) {
-      if (M.grp[k] == M.grp[k + 1] && f[k] < f[k + 1]) {
-        const float t = f[k];
-        f[k] = f[k + 1];
-        f[k + 1] = t;
-      }
-    }
-  }
+  for (int k = NPR; k < MLP_MAXP; ++k) f[k] = 0.0f;
 }
 
 // normalised features 8g..8g+7 of one walker as a B operand chunk (D = NPR features)
@@ -485,7 +562,7 @@ __global__ void k_mlp_desc(MlpNet M, const R* x, long long n, float* FT) {
   mlp_descriptor<R, NP>(M, xx, f);
 #pragma unroll
   for (int k = 0; k < MLP_MAXP; ++k)
-    if (k < M.npairs) FT[(long long)k * n + w] = f[k];
+    if (k < M.D) FT[(long long)k * n + w] = f[k];
 }
 
 // raw features -> normalised features in place
@@ -523,10 +600,66 @@ long long off_W(int l, int D, int H) {
 long long off_b(int l, int D, int H) { return off_W(l, D, H) + (l == 0 ? (long long)H * D : (long long)H * H); }
 long long off_w3(int D, int H) { return (long long)H * D + 3ll * H + 2ll * H * H; }
 
+// the atom-sorting tables of the descriptor from the DistIt-style specification; n_atoms = 0
+// describes a network trained on given features only (no descriptor)
+bool build_sorting(const pvd_mlp* m, MlpNet& N) {
+  const int na = m->n_atoms;
+  N.na = na;
+  if (na == 0) return true;
+  if (na < 2 || na > MLP_MAXA || m->D != na * (na - 1) / 2) return false;
+  int q = 0;
+  for (int i = 0; i < na; ++i) {
+    if (m->z[i] < 1) return false;
+    const double d = 0.5 * std::pow((double)m->z[i], 2.4);
+    N.cd2[i] = (float)(d * d);
+    for (int j = i + 1; j < na; ++j) N.zz[q++] = (float)m->z[i] * (float)m->z[j];
+  }
+  // sorted atoms: sublists partition the atoms (or there are none)
+  if (m->n_sorted_atoms < 0 || m->n_sorted_atoms > na) return false;
+  if (m->n_sorted_atoms > 0) {
+    int seen = 0, o = 0;
+    for (int s = 0; s < m->n_sorted_atoms; ++s) {
+      const int len = m->sorted_atoms_len[s];
+      if (len < 1 || o + len > na) return false;
+      int mask = 0;
+      for (int t = 0; t < len; ++t) {
+        const int a = m->sorted_atoms[o + t];
+        if (a < 0 || a >= na || ((seen >> a) & 1)) return false;
+        seen |= 1 << a;
+        mask |= 1 << a;
+      }
+      for (int t = 0; t < len; ++t) {
+        const int a = m->sorted_atoms[o + t];
+        N.cmask[a] = len > 1 ? (mask & ~(1 << a)) : 0;
+        for (int r = 0; r < len; ++r) N.slot[a][r] = m->sorted_atoms[o + r];
+      }
+      o += len;
+    }
+    if (seen != (1 << na) - 1) return false;
+  }
+  // sorted groups: n_groups groups of group_size distinct positions
+  N.ng = m->n_groups;
+  N.gs = m->group_size;
+  if (N.ng > 1) {
+    if (N.gs < 1 || N.ng * N.gs > na) return false;
+    int seen = 0;
+    for (int g = 0; g < N.ng; ++g)
+      for (int k = 0; k < N.gs; ++k) {
+        const int p = m->sorted_groups[g * N.gs + k];
+        if (p < 0 || p >= na || ((seen >> p) & 1)) return false;
+        seen |= 1 << p;
+        N.gpos[g][k] = p;
+      }
+  } else {
+    N.ng = 0;
+  }
+  return true;
+}
+
 bool build_net(const pvd_mlp* m, MlpNet& N) {
   std::memset(&N, 0, sizeof(N));
   const int D = m->D, H = m->H;
-  if (D < 1 || D > 32 || (H != 128 && H != 256) || m->npairs > 15 || m->npairs < D) return false;
+  if (D < 1 || D > 32 || (H != 128 && H != 256)) return false;
   N.D = D;
   N.H = H;
   const float* p = (const float*)m->params;
@@ -540,23 +673,8 @@ bool build_net(const pvd_mlp* m, MlpNet& N) {
   N.Wf[2] = pk + s0 + s1;
   N.Wt[0] = pk + s0 + 2 * s1;
   N.Wt[1] = pk + s0 + 3 * s1;
-  N.npairs = m->npairs;
   N.use_range = m->use_range;
-  for (int k = 0; k < 15; ++k) {
-    N.pi[k] = m->pair_i[k];
-    N.pj[k] = m->pair_j[k];
-    N.grp[k] = m->pair_group[k];
-    N.cidx[k] = 0;
-  }
-  // the descriptor holds every atom pair once: n_atoms = (1 + sqrt(1 + 8 npairs)) / 2
-  int na = 2;
-  while (na * (na - 1) / 2 < m->npairs) ++na;
-  if (na * (na - 1) / 2 != m->npairs) return false;
-  for (int k = 0; k < m->npairs; ++k) {
-    const int i = std::min(m->pair_i[k], m->pair_j[k]), j = std::max(m->pair_i[k], m->pair_j[k]);
-    if (i < 0 || j >= na || i == j) return false;
-    N.cidx[k] = i * na - i * (i + 1) / 2 + (j - i - 1);
-  }
+  if (!build_sorting(m, N)) return false;
   for (int k = 0; k < 32; ++k) {
     N.xmean[k] = m->xmean[k];
     N.xinv[k] = m->xinv[k];
@@ -990,8 +1108,8 @@ int pvd_mlp_descriptors(const pvd_mlp* m, int32_t precision, int32_t n_atoms, co
                         int32_t normalise, float* features_t, void* stream) {
   MlpNet N;
   if (!m || !coords || !features_t || !build_net(m, N)) return mfail(7, "bad network description");
-  if (n_atoms * (n_atoms - 1) / 2 != m->npairs || m->D != m->npairs)
-    return mfail(7, "the descriptor holds every atom pair of the walker once");
+  if (n_atoms != m->n_atoms || n_atoms < 2)
+    return mfail(7, "the network's descriptor is for another number of atoms");
   if (n <= 0) return 0;
   hipStream_t st = (hipStream_t)stream;
   const unsigned grid = (unsigned)((n + 255) / 256);
@@ -1016,8 +1134,8 @@ int pvd_mlp_energy(const pvd_mlp* m, int32_t precision, int32_t n_atoms, const v
                    const int32_t* n_dev, int64_t n_max, double* v_out, const pvd_pes_desc* fallback, void* stream) {
   MlpNet N;
   if (!m || !planes || !v_out || !build_net(m, N)) return mfail(7, "bad network description");
-  if (n_atoms * (n_atoms - 1) / 2 != m->npairs || m->D != m->npairs)
-    return mfail(7, "the descriptor holds every atom pair of the walker once");
+  if (n_atoms != m->n_atoms || n_atoms < 2)
+    return mfail(7, "the network's descriptor is for another number of atoms");
   if (n_max <= 0) return 0;
   PesParams fb;
   std::memset(&fb, 0, sizeof(fb));

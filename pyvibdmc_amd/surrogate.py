@@ -1,4 +1,4 @@
-"""Learned surrogate potential: sorted inverse-distance descriptor + MLP, trained and
+"""Learned surrogate potential: permutation-sorted inverse-distance descriptor + MLP, trained and
 evaluated by the library's BF16-MFMA kernels.
 
 The reference's learned-PES convention is ``NN_Potential`` (a model handed to a user
@@ -18,13 +18,14 @@ import numpy as np
 from . import _native as N
 from .potentials import DevicePotential
 
-__all__ = ['MLPDesc', 'pair_groups', 'SurrogateModel', 'SurrogatePotential']
+__all__ = ['MLPDesc', 'like_atoms', 'SurrogateModel', 'SurrogatePotential']
 
 
 class MLPDesc(ctypes.Structure):
-    _fields_ = [('D', ctypes.c_int32), ('H', ctypes.c_int32), ('npairs', ctypes.c_int32),
-                ('pair_i', ctypes.c_int32 * 15), ('pair_j', ctypes.c_int32 * 15), ('pair_group', ctypes.c_int32 * 15),
-                ('use_range', ctypes.c_int32),
+    _fields_ = [('D', ctypes.c_int32), ('H', ctypes.c_int32), ('n_atoms', ctypes.c_int32),
+                ('z', ctypes.c_int32 * 6), ('sorted_atoms', ctypes.c_int32 * 6), ('sorted_atoms_len', ctypes.c_int32 * 6),
+                ('n_sorted_atoms', ctypes.c_int32), ('sorted_groups', ctypes.c_int32 * 6), ('n_groups', ctypes.c_int32),
+                ('group_size', ctypes.c_int32), ('use_range', ctypes.c_int32),
                 ('xmean', ctypes.c_float * 32), ('xinv', ctypes.c_float * 32), ('lo', ctypes.c_float * 32),
                 ('hi', ctypes.c_float * 32), ('ymean', ctypes.c_float), ('ystd', ctypes.c_float),
                 ('quad', ctypes.c_float * 136), ('params', ctypes.c_void_p), ('packs', ctypes.c_void_p)]
@@ -72,43 +73,85 @@ def _check(rc, what):
         raise RuntimeError(f'{what}: {_lib().pvd_mlp_last_error().decode()}')
 
 
-def pair_groups(elements):
-    """All atom pairs (i < j), grouped by unordered element pair in order of first
-    appearance; returns (pairs, group id per pair).  Within a group the descriptor is
-    sorted, which makes it invariant to permutations of like atoms."""
-    pairs = [(i, j) for i in range(len(elements)) for j in range(i + 1, len(elements))]
-    keys = []
-    for i, j in pairs:
-        k = tuple(sorted((elements[i], elements[j])))
-        if k not in keys:
-            keys.append(k)
-    out, grp = [], []
-    for gi, k in enumerate(keys):
-        for (i, j) in pairs:
-            if tuple(sorted((elements[i], elements[j]))) == k:
-                out.append((i, j))
-                grp.append(gi)
-    return out, grp
+_Z = {e: z for z, e in enumerate('H He Li Be B C N O F Ne Na Mg Al Si P S Cl Ar'.split(), start=1)}
+_Z.update(D=1, T=1)
+
+
+def nuclear_charges(elements):
+    """Nuclear charges of element symbols (isotopes D and T are hydrogen)."""
+    try:
+        return [_Z[e] for e in elements]
+    except KeyError as err:
+        raise ValueError(f'no nuclear charge for {err.args[0]!r}: pass zs explicitly') from None
+
+
+def like_atoms(elements):
+    """DistIt ``sorted_atoms`` that sorts every element's atoms among themselves: one sublist
+    per element, in order of first appearance (a sublist of one atom keeps it in place)."""
+    out = []
+    for e in dict.fromkeys(elements):
+        out.append([i for i, x in enumerate(elements) if x == e])
+    return out
+
+
+def _sort_spec(d, n_atoms, sorted_atoms, sorted_groups):
+    """Fill the descriptor's sorting tables, with DistIt's argument checks
+    (distance_descriptors.py:55-63, :133-152)."""
+    d.n_sorted_atoms = 0
+    d.n_groups = 0
+    d.group_size = 0
+    if sorted_atoms is not None:
+        flat = [int(a) for sub in sorted_atoms for a in sub]
+        if len(flat) != n_atoms or sorted(flat) != list(range(n_atoms)):
+            raise ValueError("Please put all atoms in sorted_atoms list")
+        o = 0
+        for k, sub in enumerate(sorted_atoms):
+            d.sorted_atoms_len[k] = len(sub)
+            for a in sub:
+                d.sorted_atoms[o] = int(a)
+                o += 1
+        d.n_sorted_atoms = len(sorted_atoms)
+    if sorted_groups is not None and len(sorted_groups) > 1:
+        size = len(sorted_groups[0])
+        flat = [int(p) for g in sorted_groups for p in g]
+        if any(len(g) != size for g in sorted_groups) or len(set(flat)) != len(flat) or \
+                not all(0 <= p < n_atoms for p in flat):
+            raise ValueError('sorted_groups must be equal-size groups of distinct atom positions')
+        for k, p in enumerate(flat):
+            d.sorted_groups[k] = p
+        d.n_groups, d.group_size = len(sorted_groups), size
 
 
 class SurrogateModel:
-    """D -> H -> H -> H -> 1 SiLU network on the sorted inverse pair distances of
-    ``elements`` (H = 128 or 256), with FP32 master parameters and BF16 MFMA packs on the GPU."""
+    """D -> H -> H -> H -> 1 SiLU network (H = 128 or 256) on the inverse distances of all
+    atom pairs of ``elements`` after DistIt's permutation sorting
+    (simulation_utilities/tensorflow_descriptors/distance_descriptors.py:115-152, sort key the
+    Coulomb-matrix row norm): ``sorted_atoms`` (default: like atoms sorted among themselves,
+    ``like_atoms``), ``sorted_groups`` (default none) and ``zs`` (default: the elements'
+    nuclear charges) have DistIt's meaning.  FP32 master
+    parameters and BF16 MFMA packs on the GPU."""
 
-    def __init__(self, elements, hidden=128, seed=0):
+    def __init__(self, elements, hidden=128, seed=0, sorted_atoms='like', sorted_groups=None, zs=None):
         import torch
         self.elements = list(elements)
-        pairs, grp = pair_groups(self.elements)
-        if len(pairs) > 15:
-            raise ValueError('at most 15 atom pairs (6 atoms)')
         self.n_atoms = len(self.elements)
-        self.D = len(pairs)
+        if not 2 <= self.n_atoms <= 6:
+            raise ValueError('the descriptor supports 2 to 6 atoms')
+        if isinstance(sorted_atoms, str) and sorted_atoms == 'like':
+            sorted_atoms = like_atoms(self.elements)
+        self.sorted_atoms = None if sorted_atoms is None else [list(map(int, s)) for s in sorted_atoms]
+        self.sorted_groups = None if sorted_groups is None else [list(map(int, g)) for g in sorted_groups]
+        self.zs = [int(z) for z in (nuclear_charges(self.elements) if zs is None else zs)]
+        if len(self.zs) != self.n_atoms or min(self.zs) < 1:
+            raise ValueError('zs: one positive nuclear charge per atom')
+        self.D = self.n_atoms * (self.n_atoms - 1) // 2
         self.H = hidden
         lib = _lib()
         d = MLPDesc()
-        d.D, d.H, d.npairs = self.D, hidden, len(pairs)
-        for k, ((i, j), g) in enumerate(zip(pairs, grp)):
-            d.pair_i[k], d.pair_j[k], d.pair_group[k] = i, j, g
+        d.D, d.H, d.n_atoms = self.D, hidden, self.n_atoms
+        for k, z in enumerate(self.zs):
+            d.z[k] = z
+        _sort_spec(d, self.n_atoms, self.sorted_atoms, self.sorted_groups)
         for k in range(32):
             d.xmean[k] = 0.0
             d.xinv[k] = 1.0

@@ -1,6 +1,11 @@
-"""Pins of the surrogate oracle: backward pass by central differences, descriptor
-invariances and closed-form distances, bfloat16 rounding, Adam's first step."""
+"""Pins of the surrogate oracle: backward pass by central differences, the descriptor against
+vectors the reference's DistIt wrote, its permutation invariance on the reference test's
+fixtures and closed-form distances, bfloat16 rounding, Adam's first step."""
+import json
+import os
+
 import numpy as np
+import pytest
 
 from oracle import mlp as O
 from oracle import pes as P
@@ -16,23 +21,76 @@ def test_bf16_rounding_is_nearest_even():
     assert np.all((r.view(np.uint32) & 0xFFFF) == 0)
 
 
-def test_descriptor_closed_form_and_permutation_invariance():
+GOLDEN = os.path.join(os.path.dirname(__file__), 'golden', 'distit.npz')
+
+
+def golden_cases():
+    g = np.load(GOLDEN)
+    for name in sorted({k.split('/')[0] for k in g.files}):
+        spec = json.loads(str(g[name + '/spec']))
+        yield name, g[name + '/cds'], g[name + '/zs'], spec['sorted_atoms'], spec['sorted_groups'], g[name + '/coulomb']
+
+
+def coulomb_scale(zs):
+    m = len(zs)
+    return np.array([zs[i] * zs[j] for i in range(m) for j in range(i + 1, m)], np.float64)
+
+
+def test_descriptor_matches_reference_distit_vectors():
+    """Vectors the reference's DistIt(method='coulomb') wrote (tests/golden/gen_distit_golden.py):
+    water and the six-atom cluster under like-atom, per-monomer, group-only and reversed-sublist
+    sorting, and the reference test's trimer fixtures.  The oracle's features are those values
+    divided by z_i z_j."""
+    n = 0
+    for name, cds, zs, sa, sg, ref in golden_cases():
+        got = O.descriptor(cds, zs, sa, sg) * coulomb_scale(zs)
+        np.testing.assert_allclose(got, ref, rtol=1e-14, atol=0, err_msg=name)
+        n += 1
+    assert n == 6
+
+
+def test_descriptor_permutation_invariance_of_reference_fixtures():
+    """pyvibdmc/tests/test_cupy_dist.py:80-142: swapping two waters and the hydrogens of one
+    leaves the monomer-sorted descriptor exactly unchanged (count_nonzero of the difference)."""
+    for name, cds, zs, sa, sg, _ in golden_cases():
+        if name != 'trimer_reference_fixture':
+            continue
+        d = O.descriptor(cds, zs, sa, sg)
+        assert np.count_nonzero(np.diff(d[:4], axis=0)) == 0
+        assert np.count_nonzero(np.diff(d[4:], axis=0)) == 0
+
+
+def test_descriptor_closed_form_for_water():
+    """[H, H, O] with the hydrogens sorted: the H nearer to O has the larger Coulomb row norm,
+    so the features are (1/r_HH, max 1/r_OH, min 1/r_OH)."""
     x = P.water_eq()[None] + np.random.RandomState(0).normal(0, 0.1, (50, 3, 3))
-    pairs, sizes = O.pair_groups(['H', 'H', 'O'])
-    assert pairs == [(0, 1), (0, 2), (1, 2)] and sizes == [1, 2]
-    d = O.descriptor(x, pairs, sizes)
+    d = O.descriptor(x, [1, 1, 8], O.like_atoms(['H', 'H', 'O']))
     r1 = np.linalg.norm(x[:, 0] - x[:, 2], axis=1)
     r2 = np.linalg.norm(x[:, 1] - x[:, 2], axis=1)
+    np.testing.assert_allclose(d[:, 0], 1 / np.linalg.norm(x[:, 0] - x[:, 1], axis=1), rtol=1e-14)
     np.testing.assert_allclose(d[:, 1], np.maximum(1 / r1, 1 / r2), rtol=1e-14)
     np.testing.assert_allclose(d[:, 2], np.minimum(1 / r1, 1 / r2), rtol=1e-14)
-    swapped = x[:, [1, 0, 2]]
-    np.testing.assert_array_equal(O.descriptor(swapped, pairs, sizes), d)
-    # six atoms: swapping the monomers and the hydrogens inside one leaves it unchanged
-    c = P.cluster6_eq()[None] + np.random.RandomState(1).normal(0, 0.1, (20, 6, 3))
-    pairs6, sizes6 = O.pair_groups(['O', 'H', 'H', 'O', 'H', 'H'])
-    assert len(pairs6) == 15 and sorted(sizes6) == [1, 6, 8]
-    d6 = O.descriptor(c, pairs6, sizes6)
-    np.testing.assert_allclose(O.descriptor(c[:, [3, 4, 5, 0, 2, 1]], pairs6, sizes6), d6, rtol=1e-14)
+    np.testing.assert_array_equal(O.descriptor(x[:, [1, 0, 2]], [1, 1, 8], [[0, 1], [2]]), d)
+
+
+@pytest.mark.parametrize('mistake', ['ascending', 'squared_group_norms', 'unstable_groups', 'no_group_sort'])
+def test_planted_descriptor_mistakes_fail_the_golden_vectors(monkeypatch, mistake):
+    real = O.atom_order
+
+    def wrong(norms, sorted_atoms=None, sorted_groups=None):
+        if mistake == 'ascending':
+            return real(-norms, sorted_atoms, sorted_groups)
+        if mistake == 'squared_group_norms':
+            return real(norms ** 2, sorted_atoms, sorted_groups)
+        if mistake == 'unstable_groups':
+            return real(norms, sorted_atoms, None if sorted_groups is None else sorted_groups[::-1])
+        return real(norms, sorted_atoms, None)
+    monkeypatch.setattr(O, 'atom_order', wrong)
+    bad = 0
+    for name, cds, zs, sa, sg, ref in golden_cases():
+        got = O.descriptor(cds, zs, sa, sg) * coulomb_scale(zs)
+        bad += not np.allclose(got, ref, rtol=1e-12, atol=0)
+    assert bad > 0
 
 
 def test_backward_matches_central_differences():
