@@ -45,14 +45,14 @@ def log(*a):
     print(*a, file=sys.stderr, flush=True)
 
 
-def make_sim(n, steps, precision='fp32', imp=False, seed=1234, pot=None, train_every=None):
+def make_sim(n, steps, precision='fp32', imp=False, seed=1234, pot=None, train_every=None, dt=5):
     import pyvibdmc_amd as pv
     from pyvibdmc_amd.impsamp import water_trial
     t = np.deg2rad(104.5)
     eq = np.array([[1.81, 0.0, 0.0], [1.81 * np.cos(t), 1.81 * np.sin(t), 0.0], [0.0, 0.0, 0.0]])
     start = eq[None] + np.random.RandomState(seed).normal(0.0, 0.05, size=(n, 3, 3))
     sim = pv.DMC_Sim(num_walkers=n, num_timesteps=steps, atoms=['H', 'H', 'O'], masses=[1837.15, 1837.15, 29156.95],
-                     delta_t=5, potential=pot if pot is not None else pv.MorseBendTriatomic(), start_structures=start,
+                     delta_t=dt, potential=pot if pot is not None else pv.MorseBendTriatomic(), start_structures=start,
                      imp_samp=water_trial() if imp else None, precision=precision, rng='philox', seed=seed,
                      write_files=False, equil_steps=10 ** 9, wfn_every=10 ** 9, chkpt_every=10 ** 9,
                      DEBUG_save_training_every=train_every)
@@ -83,15 +83,46 @@ def zpe_of(vref, t=None):
     return float(tail.mean()), float(blocking_se(tail))
 
 
-def time_to_accuracy(vref, target, ms_per_step, tol=1.0, every=250):
-    """First step count t (multiple of `every`) at which the running ZPE is within `tol`
-    cm^-1 of `target` with a blocking error below `tol`, and the wall time it costs at the
-    measured step rate."""
-    for t in range(every, len(vref) + 1, every):
-        z, se = zpe_of(vref, t)
-        if abs(z - target) <= tol and se <= tol:
-            return {"steps": t, "seconds": t * ms_per_step * 1e-3}
+def time_to_accuracy(vref, target, ms_per_step, tol=1.0, sigma=0.3, every=250):
+    """First checkpoint t (a multiple of `every`) from which the running ZPE estimate stays
+    within `tol` cm^-1 of `target` up to the end of the run and whose blocking error is at
+    most `sigma`, with the wall time it costs at the measured step rate; None if the run never
+    settles there.  Staying, not touching, is required: an estimate that crosses the band
+    because of its time-step bias or a fluctuation has not converged."""
+    v = np.asarray(vref, np.float64) / WN
+    ts = np.arange(every, len(v) + 1, every)
+    c = np.concatenate([[0.0], np.cumsum(v)])
+    lo = ts // 4
+    means = (c[ts] - c[lo]) / (ts - lo)
+    inside = np.abs(means - target) <= tol
+    if not inside[-1]:
+        return None
+    k = len(ts) - 1
+    while k > 0 and inside[k - 1]:
+        k -= 1
+    for j in range(k, len(ts)):
+        z, se = zpe_of(vref, int(ts[j]))
+        if se <= sigma:
+            t = int(ts[j])
+            return {"steps": t, "seconds": t * ms_per_step * 1e-3, "zpe_cm": z, "se_cm": se}
     return None
+
+
+def extrapolate_dt0(dts, zs, ses):
+    """ZPE(dt) = a + b dt + c dt^2 through three time steps: a and its standard error (the
+    Lagrange weights at dt = 0 applied to the independent runs' errors), plus a + b dt fitted
+    by weighted least squares to the two smallest steps as a check of the curvature."""
+    dts = np.asarray(dts, np.float64)
+    zs = np.asarray(zs, np.float64)
+    ses = np.asarray(ses, np.float64)
+    w = np.array([np.prod([(0.0 - dts[j]) / (dts[i] - dts[j]) for j in range(3) if j != i]) for i in range(3)])
+    a = float(w @ zs)
+    se = float(np.sqrt(np.sum((w * ses) ** 2)))
+    o = np.argsort(dts)[:2]
+    lin = float((zs[o[0]] * dts[o[1]] - zs[o[1]] * dts[o[0]]) / (dts[o[1]] - dts[o[0]]))
+    lin_se = float(np.hypot(ses[o[0]] * dts[o[1]], ses[o[1]] * dts[o[0]]) / (dts[o[1]] - dts[o[0]]))
+    return {"zpe0_cm": a, "se_cm": se, "lagrange_weights": w.tolist(), "linear_two_smallest_cm": lin,
+            "linear_se_cm": lin_se}
 
 
 def timed_propagate(sim):
@@ -214,48 +245,110 @@ def continuous_variant(n=100_000, T=10_000):
     return out
 
 
-def zpe_study(n, T, rates, model):
-    """ZPE runs of T steps at n walkers: plain and importance-sampled DMC in FP32, their
-    FP64 runs (the oracles the FP32 forms are judged against), and the surrogate form.
-    Each ZPE comes with its blocking error; each FP32 form also reports the wall time, at its
-    measured throughput, after which its running ZPE is within 1 cm^-1 of the FP64 run of the
-    same form with an error bar below 1 cm^-1."""
+DTS = (5.0, 2.5, 1.25)
+TAU = 100_000.0  # imaginary time of every extrapolation run (a.u.)
+
+
+def zpe_study(n, T, model):
+    """The FP64 oracle and the FP32 forms against it, at n walkers.
+
+    * FP64 plain and FP64 importance-sampled DMC at dt = 5, 2.5 and 1.25 over the same
+      imaginary time (TAU), each extrapolated to dt -> 0 (``extrapolate_dt0``).  The two
+      propagators carry different time-step errors but share the dt -> 0 limit, so their
+      agreement checks the importance-sampled propagator, and the combined limit is the
+      oracle ZPE.
+    * FP32 plain, importance-sampled and surrogate runs at dt = 5 (T steps), each against the
+      FP64 run of the same form and time step."""
     import torch
     import pyvibdmc_amd as pv
     from pyvibdmc_amd.surrogate import SurrogatePotential
-    runs = [('plain_fp64', 'fp64', False, None), ('plain_fp32', 'fp32', False, None),
-            ('is_fp64', 'fp64', True, None), ('is_fp32', 'fp32', True, None)]
+    out = {"walkers": n, "estimator": "mean of vref over steps T/4..T (pyvibdmc.py:904-908), blocking error",
+           "imaginary_time_au": TAU}
+    fp64 = {}
+    for form, imp in (('plain', False), ('is', True)):
+        rows = []
+        for dt in DTS:
+            steps = int(round(TAU / dt))
+            log(f'[bench] FP64 {form} dt={dt}: {n} walkers x {steps} steps')
+            sim = make_sim(n, steps, precision='fp64', imp=imp, seed=2024, dt=dt)
+            wall, ws = timed_propagate(sim)
+            z, se = zpe_of(sim._vref_vs_tau)
+            rows.append({"dt": dt, "steps": steps, "zpe_cm": z, "se_cm": se, "wall_s": wall})
+            log(f'[bench]   ZPE {z:.2f} +- {se:.2f} cm-1 in {wall:.1f} s')
+            del sim
+            torch.cuda.empty_cache()
+        ext = extrapolate_dt0([r['dt'] for r in rows], [r['zpe_cm'] for r in rows], [r['se_cm'] for r in rows])
+        fp64[form] = {"runs": rows, "dt0": ext}
+        log(f'[bench] FP64 {form}: dt->0 {ext["zpe0_cm"]:.2f} +- {ext["se_cm"]:.2f} cm-1')
+    a, b = fp64['plain']['dt0'], fp64['is']['dt0']
+    diff = a['zpe0_cm'] - b['zpe0_cm']
+    comb = np.hypot(a['se_cm'], b['se_cm'])
+    wa, wb = 1 / a['se_cm'] ** 2, 1 / b['se_cm'] ** 2
+    oracle = (wa * a['zpe0_cm'] + wb * b['zpe0_cm']) / (wa + wb)
+    out['fp64'] = fp64
+    out['dt0_plain_minus_is_cm'] = diff
+    out['dt0_plain_minus_is_sigma'] = diff / comb
+    out['oracle_zpe_cm'] = float(oracle)
+    out['oracle_se_cm'] = float(1 / np.sqrt(wa + wb))
+    # FP32 forms at dt = 5 against the FP64 run of the same form and step
+    runs = [('plain_fp32', False, None), ('is_fp32', True, None)]
     if model is not None:
-        runs.append(('is_surrogate_fp32', 'fp32', True, 'surrogate'))
-    out = {"steps": T, "walkers": n, "dt": 5, "estimator": "mean of vref over steps T/4..T (pyvibdmc.py:904-908)"}
-    series = {}
-    for name, prec, imp, kind in runs:
+        runs.append(('is_surrogate_fp32', True, 'surrogate'))
+    for name, imp, kind in runs:
         pot = SurrogatePotential(model, pv.MorseBendTriatomic(), judge_every=100) if kind else None
-        log(f'[bench] ZPE run {name}: {n} walkers x {T} steps')
-        sim = make_sim(n, T, precision=prec, imp=imp, seed=2024, pot=pot)
+        log(f'[bench] {name} dt=5: {n} walkers x {T} steps')
+        sim = make_sim(n, T, precision='fp32', imp=imp, seed=2024, pot=pot)
         wall, ws = timed_propagate(sim)
-        series[name] = sim._vref_vs_tau.copy()
-        z, se = zpe_of(series[name])
-        out[name] = {"zpe_cm": z, "se_cm": se, "wall_s": wall, "walker_steps_per_s": ws / wall}
+        z, se = zpe_of(sim._vref_vs_tau)
+        same = fp64['is' if imp else 'plain']['runs'][0]
+        out[name] = {"dt": 5.0, "steps": T, "zpe_cm": z, "se_cm": se, "wall_s": wall, "walker_steps_per_s": ws / wall,
+                     "err_vs_fp64_same_form_cm": z - same['zpe_cm'] if same['steps'] == T else None,
+                     "err_vs_oracle_cm": z - oracle}
         log(f'[bench]   {name}: ZPE {z:.2f} +- {se:.2f} cm-1 in {wall:.1f} s')
         del sim
         torch.cuda.empty_cache()
-    zp = out['plain_fp64']['zpe_cm']
-    for name, oracle in (('plain_fp32', 'plain_fp64'), ('is_fp32', 'is_fp64'), ('is_surrogate_fp32', 'is_fp64')):
-        if name in out:
-            out[name]['err_vs_fp64_same_form_cm'] = out[name]['zpe_cm'] - out[oracle]['zpe_cm']
-            out[name]['err_vs_plain_fp64_cm'] = out[name]['zpe_cm'] - zp
-    # plain and importance-sampled DMC carry different time-step errors at dt = 5, so each
-    # FP32 form is timed to 1 cm^-1 of the FP64 run of its own form
-    oracle_of = {'plain_fp32': 'plain_fp64', 'is_fp32': 'is_fp64', 'is_surrogate_fp32': 'is_fp64'}
-    tta = {}
-    for name, ms in rates.items():
-        if name in series and ms:
-            tta[name] = time_to_accuracy(series[name], out[oracle_of[name]]['zpe_cm'], ms)
-    out['time_to_1cm_vs_fp64_same_form'] = tta
-    out['dt_bias_is_minus_plain_fp64_cm'] = out['is_fp64']['zpe_cm'] - zp
-    a, b = tta.get('plain_fp32'), tta.get('is_surrogate_fp32')
-    out['speedup_is_surrogate_over_plain'] = (a['seconds'] / b['seconds']) if (a and b) else None
+    return out
+
+
+def time_to_accuracy_study(model, oracle, n=100_000, tol=1.0, sigma=0.3):
+    """Wall time for each FP32 form to settle within tol cm^-1 of the oracle with a blocking
+    error <= sigma (``time_to_accuracy``), in the variance-limited regime (n walkers): every
+    form runs at each time step of DTS (the importance-sampled forms also at 0.625, since their
+    time-step bias at 1.25 is about 1 cm^-1) for a fixed imaginary time long enough for plain
+    DMC to reach sigma; a form's time is that of its fastest time step.  The ratio plain /
+    surrogate is the measured speed-up."""
+    import torch
+    import pyvibdmc_amd as pv
+    from pyvibdmc_amd.surrogate import SurrogatePotential
+    forms = [('plain_fp32', False, None, 500_000.0, DTS), ('is_fp32', True, None, 100_000.0, DTS + (0.625,))]
+    if model is not None:
+        forms.append(('is_surrogate_fp32', True, 'surrogate', 100_000.0, DTS + (0.625,)))
+    out = {"walkers": n, "tol_cm": tol, "sigma_cm": sigma, "oracle_zpe_cm": oracle, "checkpoint_every": 250}
+    for name, imp, kind, tau, dts in forms:
+        rows = []
+        for dt in dts:
+            steps = int(round(tau / dt))
+            pot = SurrogatePotential(model, pv.MorseBendTriatomic(), judge_every=100) if kind else None
+            sim = make_sim(n, steps, precision='fp32', imp=imp, seed=77, pot=pot, dt=dt)
+            wall, _ = timed_propagate(sim)
+            ms = wall * 1e3 / steps
+            z, se = zpe_of(sim._vref_vs_tau)
+            tta = time_to_accuracy(sim._vref_vs_tau, oracle, ms, tol, sigma)
+            rows.append({"dt": dt, "steps_run": steps, "ms_per_step": ms, "zpe_cm": z, "se_cm": se,
+                         "err_vs_oracle_cm": z - oracle, "reached": tta})
+            log(f'[bench] TTA {name} dt={dt}: {ms * 1e3:.1f} us/step, full-run ZPE {z:.2f} +- {se:.2f}, '
+                f'reached {tta}')
+            del sim
+            torch.cuda.empty_cache()
+        ok = [r for r in rows if r['reached']]
+        best = min(ok, key=lambda r: r['reached']['seconds']) if ok else None
+        out[name] = {"runs": rows, "best": None if best is None else
+                     {"dt": best['dt'], "steps": best['reached']['steps'], "seconds": best['reached']['seconds']}}
+    p, q = out['plain_fp32']['best'], (out.get('is_surrogate_fp32') or {}).get('best')
+    out['speedup_is_surrogate_over_plain'] = (p['seconds'] / q['seconds']) if (p and q) else None
+    r = (out.get('is_fp32') or {}).get('best')
+    out['speedup_is_over_plain'] = (p['seconds'] / r['seconds']) if (p and r) else None
+    out['target_speedup'] = 3.0
     return out
 
 
@@ -347,6 +440,9 @@ def main():
         roof = {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                 "frac": achieved / HBM_PEAK_GBS,
                 "traffic": load_traffic(f'k_discrete_{args.precision}_np9_{n}'),
+                "traffic_source": "committed counter profile (profiles/k_discrete_water_1M_walkers_hbm_traffic.txt: "
+                                  "rocprofv3 --pmc FETCH_SIZE, then WRITE_SIZE, scaled by a copy kernel of known "
+                                  "size); read from profiles/traffic.json, not measured in this run",
                 "kernel": f"k_discrete<{rname},9> (fused move+PES+branch+compaction)",
                 "avg_launch_us": k_ms * 1e3, "bytes_per_launch": bytes_per_launch,
                 "bytes_per_walker_step": bpw,
@@ -385,8 +481,13 @@ def main():
         model, tinfo = train_surrogate(n, args.epochs, args.train_geoms)
         log(f'[bench] surrogate trained in {tinfo["train_seconds"]:.1f} s, rms {tinfo["rms_err_cm"]:.2f} cm-1')
         spot = SurrogatePotential(model, pv.MorseBendTriatomic(), judge_every=100, judge_stride=100)
-        sim = make_sim(n, W + K + 2, imp=True, pot=spot)
+        extra = max(0, 101 - (W + K))   # untimed steps after the timing, up to the first judge record
+        sim = make_sim(n, W + K + extra + 2, imp=True, pot=spot)
         ws3, wall3, mean3, _, _ = run_timed(sim, W, K, 0)
+        if extra:
+            from pyvibdmc_amd import _native as N
+            N.check(N.load().pvd_sim_run(sim._h, extra, N.stream_handle()))
+            sim._raise_if_failed()
         # the network kernel alone, on the simulation's own walkers and stream
         from pyvibdmc_amd import _native as N
         import ctypes
@@ -420,11 +521,10 @@ def main():
     else:
         model = None
 
-    zpe = None
+    zpe = tta = None
     if not args.no_variants and args.zpe_steps > 0:
-        rates = {'plain_fp32': ms_per_step, 'is_fp32': (variants.get('is') or {}).get('ms_per_step'),
-                 'is_surrogate_fp32': (variants.get('is_surrogate') or {}).get('ms_per_step')}
-        zpe = zpe_study(n, args.zpe_steps, rates, model)
+        zpe = zpe_study(n, args.zpe_steps, model)
+        tta = time_to_accuracy_study(model, zpe['oracle_zpe_cm'])
 
     cpu = None
     if not args.no_cpu:
@@ -439,7 +539,7 @@ def main():
                                   f"birth/death, {args.precision} coordinates with FP64 weights/reductions, "
                                   f"Philox draws",
                       "walkers": n, "mean_population": mean_n, "parallelism": "single GPU"},
-           "roofline": roof, "cpu_baseline": cpu, "variants": variants, "zpe": zpe}
+           "roofline": roof, "cpu_baseline": cpu, "variants": variants, "zpe": zpe, "time_to_accuracy": tta}
     print(json.dumps(out), flush=True)
 
 
