@@ -461,13 +461,21 @@ def main():
     if not args.no_variants:
         log('[bench] importance-sampled variant')
         sim = make_sim(n, total, precision=args.precision, imp=True)
-        ws2, wall2, mean2, kern2, _ = run_timed(sim, W, K, min(args.event_steps, 50))
+        ws2, wall2, mean2, kern2, ev2 = run_timed(sim, W, K, min(args.event_steps, 50))
         variants['is'] = {"value": ws2 / wall2, "unit": "walker-steps/s", "ms_per_step": wall2 * 1e3 / K,
                           "config": "same PES and walkers, Gaussian-product psi_T (2 stretches + bend), drift and "
                                     "local energy from the trial kernels, Metropolis accept, dt_eff"}
         if kern2:
             names = ['k_is_move', 'k_discrete']
             variants['is']['step_kernels_us'] = {nm: t * 1e3 for nm, t in zip(names, kern2)}
+            bpw_is = BYTES_PER_WALKER_STEP[('is', args.precision, 9)] if args.precision == 'fp32' else None
+            if bpw_is:
+                step_ms = sum(kern2)
+                ach = bpw_is * ev2 / (step_ms * 1e-3) / 1e9
+                variants['is']['roofline'] = {"bound": "hbm", "achieved": ach, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                                              "frac": ach / HBM_PEAK_GBS, "bytes_per_walker_step": bpw_is,
+                                              "kernels": "k_is_move + k_discrete<float,9,IS> per step (algorithmic "
+                                                         "bytes of the step over both kernels' time)"}
         log(f'[bench] IS: {ws2 / wall2:.4g} walker-steps/s')
         del sim
         torch.cuda.empty_cache()

@@ -1,7 +1,6 @@
 // C ABI of the DMC propagation library: simulation handle, launch dispatch and the
 // auxiliary kernels (layout transposes, first-step vref, descendant weights, stand-alone
 // potential / trial evaluation).  Declarations and conventions: include/pyvibdmc_amd.h.
-#include <cstdlib>
 #include <initializer_list>
 
 #include "pvd_sim.hpp"
@@ -298,20 +297,6 @@ int pvd_sim_create(const pvd_config* cfg, pvd_sim** out) {
                                                  hipSuccess)
       s->cus = cus;
     if (s->cus < 1) s->cus = 1;
-  }
-  {
-    // test controls of the importance-sampled discrete step: PVD_IS_FUSE=0 disables the fused
-    // move + branch; PVD_IS_SLOTS=k tiles the branching kernel (fused or not) over at most k
-    // workgroups, which moves the fused step's population threshold down and keeps the tile
-    // partition (hence every FP64 sum) the same with and without fusion
-    const char* f = std::getenv("PVD_IS_FUSE");
-    s->is_fuse_off = (f && f[0] == '0' && f[1] == 0) ? 1 : 0;
-    const char* e = std::getenv("PVD_IS_SLOTS");
-    s->is_slot_cap = 0;
-    if (e && *e) {
-      const long v = std::strtol(e, nullptr, 10);
-      s->is_slot_cap = v > 0 ? (int)std::min<long>(v, 1 << 20) : 0;
-    }
   }
   // tiles of the discrete kernels: at most cap / TILE plus one round of resident workgroups
   // (at most 8 per CU) - see discrete_tiles and disc_grid

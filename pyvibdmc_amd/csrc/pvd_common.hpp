@@ -270,9 +270,6 @@ PVD_DEV void st_relaxed(uint64_t* p, uint64_t v) {
 PVD_DEV unsigned fetch_add_relaxed(unsigned* p, unsigned v) {
   return __hip_atomic_fetch_add(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
-PVD_DEV uint64_t fetch_add_relaxed64(uint64_t* p, uint64_t v) {
-  return __hip_atomic_fetch_add(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-}
 PVD_DEV void fence_release() { __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent"); }
 PVD_DEV unsigned ld_relaxed_u(const unsigned* p) {
   return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);

@@ -142,7 +142,6 @@ struct StepArgs {
   Rng rng;
   int par;
   int slots;   // discrete: resident workgroups of the step kernel (tiling unit)
-  int fuse_is;  // discrete IS: the branching kernel is the fused form (k_discrete<..., FUSE>)
   int branch;
   int ms, ds;  // atoms, dims
   int is_;     // importance sampling on
@@ -164,14 +163,6 @@ constexpr int ACC_STRIDE = 32;  // unsigned words between slots
 template <typename R>
 PVD_DEV unsigned* acc_slot(const StepArgs<R>& A, int par, int b) {
   return A.accs + (size_t)(par * ACC_SLOTS + (b & (ACC_SLOTS - 1))) * ACC_STRIDE;
-}
-
-// A slot as one 64-bit word: the low half is the accepted-move count, the high half counts
-// the tiles of the fused importance-sampled step that have finished their moves (zero on the
-// two-kernel path).
-template <typename R>
-PVD_DEV uint64_t* acc_slot64(const StepArgs<R>& A, int par, int b) {
-  return reinterpret_cast<uint64_t*>(acc_slot(A, par, b));
 }
 
 // Total accepted moves of parity `par`, on every lane of the calling wave.
@@ -634,10 +625,9 @@ PVD_DEV void finalize_discrete(const StepArgs<R>& A, Ctrl* C, int par, int n, in
       A.tr_dteff[step] = C->eff_t;
     }
   }
-  // every tile read the acceptance count (and passed the fused step's barrier) before
-  // publishing: clear both halves for step + 2
+  // every tile read the acceptance count before publishing: clear it for step + 2
   if (IS) {
-    for (int b = 0; b < ACC_SLOTS; ++b) st_relaxed(acc_slot64(A, par, b), 0ull);
+    for (int b = 0; b < ACC_SLOTS; ++b) st_relaxed_u(acc_slot(A, par, b), 0u);
   }
 }
 
@@ -645,29 +635,14 @@ PVD_DEV void finalize_discrete(const StepArgs<R>& A, Ctrl* C, int par, int n, in
 // from V_ext (walkers already moved; NP = 0 reads the component count at run time).
 // Tile b is block b.  Workgroups are dispatched in index order within each XCD, so the
 // predecessors a tile waits for are resident or finished; the wait is still bounded.
-//
-// FUSE (importance sampling with a device trial, IS and !EXT): the weights need dt_eff, i.e.
-// the acceptance count of the whole step (pyvibdmc.py:387-389, 603).  When every tile is
-// resident (ntiles <= slots, the resident workgroups of this kernel) the tile makes the
-// moves itself (k_is_move then returns at once): it keeps the moved walkers in LDS and their
-// energies in registers, adds (1 << 32 | accepted) to its acceptance slot, and waits until
-// the high halves of the slots count every tile -- a grid barrier that every tile reaches,
-// since all of them are resident; the wait is bounded and reports ST_SPIN if it expires.
-// Otherwise k_is_move has moved the walkers and this kernel reads them back.  Both paths
-// evaluate the same functions on the same draws, so their results are identical.  The host
-// picks the fused form when the nominal population fits the resident tiles; the unfused form
-// keeps the registers (and occupancy) of a kernel that only branches.
-template <typename R, int NP, bool INJ, bool IS, bool EXT, int MS = MS_ANY, bool FUSE_ = false>
+template <typename R, int NP, bool INJ, bool IS, bool EXT, int MS = MS_ANY>
 __global__ __launch_bounds__(TILE) void k_discrete(StepArgs<R> A) {
   constexpr int W = disc_wpt<R, NP, EXT, IS>();
   constexpr int TW = TILE * W;
-  constexpr bool FUSE = FUSE_ && IS && !EXT;
   __shared__ int s_prefix, s_status;
   __shared__ int s_wt[W][TILE / PVD_WAVE];
   __shared__ double s_d[TILE / PVD_WAVE];
   __shared__ int s_i[TILE / PVD_WAVE], s_i2[TILE / PVD_WAVE], s_i3[TILE / PVD_WAVE];
-  __shared__ R s_x[FUSE ? NP : 1][FUSE ? TW : 1];  // moved walkers of the fused step
-  __shared__ unsigned s_acc;
   const int par = A.par;
   Ctrl* C = A.ctrl;
   const int tile = blockIdx.x;
@@ -679,62 +654,7 @@ __global__ __launch_bounds__(TILE) void k_discrete(StepArgs<R> A) {
   const unsigned epoch = C->epoch[par];
   const double vref = C->vref[par];
   double dte = A.dt;
-  const bool fused = FUSE && ntiles <= A.slots;
-  double Ef[FUSE ? W : 1];
-  if constexpr (FUSE) {
-    if (fused) {
-      int a = 0;
-#pragma unroll
-      for (int k = 0; k < W; ++k) {
-        const int li = k * TILE + threadIdx.x;
-        const long long i = (long long)tile * tw + li;
-        Ef[k] = 0.0;
-        if (li < tw && i < n) {
-          R y[NP], disp[NP];
-          double um, ub;
-          load_walker<R, NP>(A, i, y);
-          draw_step<R, NP, INJ, true>(A, i, step, disp, um, ub, false);
-          bool acc;
-          const R ke = is_move<R, NP, INJ, MS>(A, disp, um, y, acc);
-          a += acc ? 1 : 0;
-          const R v = pes_eval_s<R, NP, MS>(A.pes, y);
-          Ef[k] = (double)(v + ke);
-#pragma unroll
-          for (int p = 0; p < NP; ++p) s_x[p][li] = y[p];
-        }
-      }
-      a = block_sum<TILE>(a, s_i);
-      if (threadIdx.x < PVD_WAVE) {
-        if (threadIdx.x == 0) fetch_add_relaxed64(acc_slot64(A, par, tile), (1ull << 32) | (uint64_t)(unsigned)a);
-        long long spins = 0;
-        bool ok = true;
-        unsigned arrived = 0, acc_sum = 0;
-        for (;;) {
-          const uint64_t v = ld_relaxed(acc_slot64(A, par, lane_id()));
-          arrived = (unsigned)wave_sum((int)(v >> 32));
-          acc_sum = (unsigned)wave_sum((int)(uint32_t)v);
-          if (arrived >= (unsigned)ntiles) break;
-          if (++spins > (1ll << 20)) {
-            ok = false;
-            break;
-          }
-          __builtin_amdgcn_s_sleep(1);
-        }
-        if (threadIdx.x == 0) {
-          if (!ok) st_release_i(&C->status, ST_SPIN);
-          s_status = ok ? 0 : 1;
-          s_acc = acc_sum;
-        }
-      }
-      __syncthreads();
-      if (s_status) return;
-      dte = A.dt * ((double)s_acc / (double)n);
-    } else {
-      dte = A.dt * ((double)acc_total(A, par) / (double)n);
-    }
-  } else if (IS) {
-    dte = A.dt * ((double)acc_total(A, par) / (double)n);
-  }
+  if (IS) dte = A.dt * ((double)acc_total(A, par) / (double)n);
 
   R x[W][EXT ? 1 : NP];
   int c[W];
@@ -751,13 +671,9 @@ __global__ __launch_bounds__(TILE) void k_discrete(StepArgs<R> A) {
         E = A.V_ext[i];
         if (IS) E = E + A.ke[i];
       } else if constexpr (IS) {
-        if (fused) {
-          E = Ef[k];
-        } else {
-          // moved, and its local energy computed, by k_is_move
-          load_walker<R, NP>(A, i, x[k]);
-          E = A.E[i];
-        }
+        // moved, and its local energy computed, by k_is_move
+        load_walker<R, NP>(A, i, x[k]);
+        E = A.E[i];
       } else {
         load_walker<R, NP>(A, i, x[k]);
         R disp[NP];
@@ -894,10 +810,10 @@ __global__ __launch_bounds__(TILE) void k_discrete(StepArgs<R> A) {
         } else if constexpr (sizeof(R) == 4) {
           const __amdgpu_buffer_rsrc_t r = coord_rsrc((const float*)A.cout, A.cap, NP);
 #pragma unroll
-          for (int p = 0; p < NP; ++p) coord_st(r, A.cap, p, dst, (FUSE && fused) ? s_x[p][li] : x[k][p]);
+          for (int p = 0; p < NP; ++p) coord_st(r, A.cap, p, dst, x[k][p]);
         } else {
 #pragma unroll
-          for (int p = 0; p < NP; ++p) A.cout[p * A.cap + dst] = (FUSE && fused) ? s_x[p][li] : x[k][p];
+          for (int p = 0; p < NP; ++p) A.cout[p * A.cap + dst] = x[k][p];
         }
         if (A.who_out) A.who_out[dst] = A.who_in[i];
         if constexpr (EXT) {
@@ -927,12 +843,6 @@ __global__ __launch_bounds__(TILE) void k_is_move(StepArgs<R> A) {
   const int ntiles = (n + TILE - 1) / TILE;
   const int tile = blockIdx.x;
   if (C->status != ST_OK || tile >= ntiles) return;
-  if (A.fuse_is) {
-    // the fused branching kernel with every tile resident makes the moves itself
-    int nt_d, tw_d;
-    discrete_tiles(n, A.slots, TILE * disc_wpt<R, NP, false, true>(), nt_d, tw_d);
-    if (nt_d <= A.slots) return;
-  }
   const unsigned long long step = C->step[par];
   const long long i = (long long)tile * TILE + threadIdx.x;
   int a = 0;

@@ -88,8 +88,6 @@ struct pvd_sim {
   const void* slot_fn[16];      // resident-workgroup counts of the discrete kernels used so far
   int slot_val[16];
   int n_slot;
-  int is_fuse_off;              // 1: no fused importance-sampled step
-  int is_slot_cap;              // > 0: the IS branching kernels tile over at most this many slots
   hipGraphExec_t graph[2];  // replayed step graphs, without / with descendant tracking
   // surrogate potential (external-potential simulations only)
   int surr_on;

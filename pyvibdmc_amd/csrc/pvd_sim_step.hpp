@@ -110,26 +110,12 @@ int step_kernels(pvd_sim* s, StepArgs<R>& A, Launch* L, int& nl) {
 #define PVD_ADD(...) PVD_STEP_SWITCH(s, add((const void*)(__VA_ARGS__), g, TILE))
   if (s->is) PVD_ADD(k_is_move<R, NP, INJ, MS>);
   if (!s->cont) {
-    auto add_disc = [&](const void* fn, int W, int cap_slots = 0) {
+    auto add_disc = [&](const void* fn, int W) {
       A.slots = disc_slots(s, fn);
-      if (cap_slots > 0) A.slots = std::min(A.slots, cap_slots);
       add(fn, disc_grid(s, A.slots, W), TILE);
     };
     if (s->is) {
-      // the fused move + branch when the nominal population fits one resident round of tiles
-      // (its barrier needs every tile resident: ntiles <= slots, checked again on the device)
-      PVD_STEP_SWITCH(s, {
-        constexpr int Wi = disc_wpt<R, NP, false, true>();
-        const void* ff = (const void*)k_discrete<R, NP, INJ, true, false, MS, true>;
-        int sl = s->is_fuse_off ? 0 : disc_slots(s, ff);
-        if (s->is_slot_cap > 0) sl = std::min(sl, s->is_slot_cap);
-        if (sl > 0 && (long long)sl * TILE * Wi >= (long long)s->cfg.num_walkers) {
-          A.fuse_is = 1;
-          add_disc(ff, Wi, s->is_slot_cap);
-        } else {
-          add_disc((const void*)k_discrete<R, NP, INJ, true, false, MS>, Wi, s->is_slot_cap);
-        }
-      });
+      PVD_STEP_SWITCH(s, add_disc((const void*)k_discrete<R, NP, INJ, true, false, MS>, disc_wpt<R, NP, false, true>()));
     } else {
       PVD_STEP_SWITCH(s, add_disc((const void*)k_discrete<R, NP, INJ, false, false, (MS & MS_PES_ANY_SPEC)>,
                                   disc_wpt<R, NP, false>()));
@@ -302,7 +288,6 @@ int run_steps_timed(pvd_sim* s, int n_steps, hipStream_t st, float* avg_ms, int3
   for (int k = 0; k < n_steps; ++k) {
     StepArgs<R> A = make_args<R>(s);
     A.slots = A0.slots;
-    A.fuse_is = A0.fuse_is;
     void* args[] = {&A};
     for (int i = 0; i < nl; ++i) {
       // start / stop events recorded by the dispatch itself (kernel execution only)
