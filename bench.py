@@ -352,6 +352,95 @@ def time_to_accuracy_study(model, oracle, n=100_000, tol=1.0, sigma=0.3):
     return out
 
 
+CLUSTER_EL = ['O', 'H', 'H', 'O', 'H', 'H']
+# algorithmic FLOP of the 15 -> 256 -> 256 -> 256 -> 1 network per walker
+MLP256_FLOP_PER_WALKER = 2 * (15 * 256 + 256 * 256 + 256 * 256 + 256)
+
+
+def cluster_sim(n, steps, precision, pot, seed, train_every=None):
+    import pyvibdmc_amd as pv
+    from oracle import pes as P
+    start = P.cluster6_eq()[None] + np.random.RandomState(seed).normal(0, 0.05, (n, 6, 3))
+    return pv.DMC_Sim(num_walkers=n, num_timesteps=steps, atoms=CLUSTER_EL,
+                      masses=[P.M_O, P.M_H, P.M_H, P.M_O, P.M_H, P.M_H], delta_t=5, potential=pot,
+                      start_structures=start, imp_samp=pv.cluster6_trial(), precision=precision, seed=seed,
+                      write_files=False, equil_steps=10 ** 9, wfn_every=10 ** 9, chkpt_every=10 ** 9,
+                      DEBUG_save_training_every=train_every)
+
+
+def cluster_study(n, steps, epochs, zpe_steps):
+    """The six-atom cluster workload: the 15 -> 256^3 -> 1 surrogate (DistIt monomer sorting)
+    trained on 2M geometries an analytic importance-sampled run visits (1M walkers, snapshots
+    at steps 300 and 600; the snapshot at step 1200 is held out), then n-walker importance-
+    sampled DMC with the analytic surface and with the surrogate: throughput over `steps`
+    timed steps, the network kernel's MFMA roofline, and (zpe_steps > 0) both ZPEs."""
+    import torch
+    import pyvibdmc_amd as pv
+    from pyvibdmc_amd.surrogate import SurrogatePotential
+    t0 = time.perf_counter()
+    s = cluster_sim(1_000_000, 1201, 'fp32', pv.TwoMonomerCluster(), 11, train_every=300)
+    s.propagate()
+    data = [(c, v) for st, c, v in s.training if st > 0]
+    coords = np.concatenate([c for c, _ in data[:2]])
+    pots = np.concatenate([v for _, v in data[:2]])
+    held_c, held_v = data[-1][0][::4], data[-1][1][::4]
+    del s, data
+    t_data = time.perf_counter() - t0
+    m = pv.SurrogateModel(CLUSTER_EL, hidden=256, seed=3, sorted_atoms=[[0], [1, 2], [3], [4, 5]],
+                          sorted_groups=[[0, 1, 2], [3, 4, 5]])
+    losses = m.fit(coords, pots, epochs=epochs, batch=8192, lr=1e-3)
+    err = (m.energy(held_c) - held_v) / WN
+    out = {"config": f"six-atom cluster, {n} walkers, importance sampling (per-monomer Gaussian psi_T), dt=5, FP32",
+           "training": {"geometries": int(len(coords)), "epochs": epochs, "batch": 8192, "lr": 1e-3,
+                        "data_seconds": t_data, "train_seconds": m.train_seconds, "final_loss": losses[-1],
+                        "held_out_rms_cm": float(np.sqrt(np.mean(err ** 2))),
+                        "held_out_max_cm": float(np.abs(err).max()), "held_out_mean_cm": float(err.mean()),
+                        "held_out": int(len(held_c))}}
+    log(f'[bench] cluster surrogate: {epochs} epochs in {m.train_seconds:.1f} s, held-out rms '
+        f'{out["training"]["held_out_rms_cm"]:.2f} cm-1')
+    del coords, pots
+    for name, mk in (('is_analytic', lambda: pv.TwoMonomerCluster()),
+                     ('is_surrogate', lambda: SurrogatePotential(m, pv.TwoMonomerCluster(), judge_every=100,
+                                                                 judge_stride=1000))):
+        pot = mk()
+        sim = cluster_sim(n, 10 + steps + 2, 'fp32', pot, 7)
+        ws, wall, mean_n, _, _ = run_timed(sim, 10, steps, 0)
+        row = {"value": ws / wall, "unit": "walker-steps/s", "ms_per_step": wall * 1e3 / steps}
+        if name == 'is_surrogate':
+            from pyvibdmc_amd import _native as N
+            import ctypes
+            planes, stride, ndev = ctypes.c_void_p(), ctypes.c_int64(), ctypes.c_void_p()
+            N.check(N.load().pvd_sim_device_view(sim._h, ctypes.byref(planes), ctypes.byref(stride),
+                                                 ctypes.byref(ndev)))
+            vbuf = torch.empty(stride.value, dtype=torch.float64, device='cuda')
+            m.energy_planes(planes.value, stride.value, ndev.value, stride.value, vbuf, N.PVD_FP32, pot.fallback)
+            ea, eb = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+            ea.record()
+            for _ in range(5):
+                m.energy_planes(planes.value, stride.value, ndev.value, stride.value, vbuf, N.PVD_FP32, pot.fallback)
+            eb.record()
+            torch.cuda.synchronize()
+            e_ms = ea.elapsed_time(eb) / 5
+            tf = MLP256_FLOP_PER_WALKER * mean_n / (e_ms * 1e-3) / 1e12
+            row["mlp_roofline"] = {"bound": "mfma", "achieved": tf, "peak": MFMA_BF16_PEAK_TFLOPS, "unit": "TFLOP/s",
+                                   "frac": tf / MFMA_BF16_PEAK_TFLOPS, "avg_launch_ms": e_ms,
+                                   "kernel": "k_mlp_energy_st<float,18,256> + k_mlp_fallback",
+                                   "flop_per_walker": MLP256_FLOP_PER_WALKER}
+            del vbuf
+        out[name] = row
+        log(f'[bench] cluster {name}: {row["value"]:.4g} walker-steps/s, {row["ms_per_step"]:.2f} ms/step')
+        del sim
+        torch.cuda.empty_cache()
+        if zpe_steps > 0:
+            sim = cluster_sim(n, zpe_steps, 'fp32', mk(), 7)
+            sim.propagate()
+            out[name]["zpe_cm"], out[name]["zpe_se_cm"] = zpe_of(sim._vref_vs_tau)
+            log(f'[bench] cluster {name}: ZPE {out[name]["zpe_cm"]:.2f} +- {out[name]["zpe_se_cm"]:.2f}')
+            del sim
+            torch.cuda.empty_cache()
+    return out
+
+
 def cpu_baseline(n):
     from oracle.cpu_ref import cpu_model, time_plain_triatomic
     steps = 60 if n >= 500_000 else 200
@@ -406,6 +495,10 @@ def main():
     ap.add_argument('--epochs', type=int, default=200)
     ap.add_argument('--train-geoms', type=int, default=2_000_000)
     ap.add_argument('--zpe-steps', type=int, default=20000, help='0 skips the ZPE runs')
+    ap.add_argument('--is-only', action='store_true', help='only the importance-sampled variant')
+    ap.add_argument('--cluster-only', action='store_true',
+                    help='only the six-atom cluster workload (16M walkers by default, --walkers to change)')
+    ap.add_argument('--cluster-zpe-steps', type=int, default=2000)
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU: the DMC walkers of this workload live on a single MI355X "
@@ -420,10 +513,21 @@ def main():
     if args.continuous_only:
         print(json.dumps({"continuous": continuous_variant()}), flush=True)
         return
+    if args.cluster_only:
+        n_c = args.walkers if args.walkers != 1_000_000 else 16_000_000
+        print(json.dumps({"cluster": cluster_study(n_c, args.steps, args.epochs, args.cluster_zpe_steps)}), flush=True)
+        return
     n = args.walkers
     K, W = args.steps, args.warmup
     total = W + K + args.event_steps + 2
 
+    if args.is_only:
+        sim = make_sim(n, total, precision=args.precision, imp=True)
+        ws2, wall2, _, kern2, _ = run_timed(sim, W, K, min(args.event_steps, 50))
+        print(json.dumps({"is": {"value": ws2 / wall2, "ms_per_step": wall2 * 1e3 / K,
+                                 "step_kernels_us": {nm: t * 1e3 for nm, t in zip(['k_is_move', 'k_discrete'],
+                                                                                  kern2 or [])}}}), flush=True)
+        return
     log(f'[bench] plain {args.precision} {n} walkers: warmup {W}, timed {K}')
     sim = make_sim(n, total, precision=args.precision)
     ws, wall, mean_n, kern_ms, ev_n = run_timed(sim, W, K, args.event_steps)
