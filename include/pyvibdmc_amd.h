@@ -123,6 +123,13 @@ int pvd_sim_run(pvd_sim* sim, int32_t n_steps, void* stream);
  * duration of the step's k-th kernel in launch order.  Synchronous.  No counterpart in the
  * reference (a measurement hook for the per-kernel timings a profiler also reports). */
 int pvd_sim_run_timed(pvd_sim* sim, int32_t n_steps, void* stream, float* avg_ms, int32_t* n_kernels);
+/* Streaming-copy ceiling of the walker layout: `reps` launches copying n walkers of np FP32
+ * planes (in, out: device, stride cap, 16-byte aligned, cap a multiple of 4) with 16-byte
+ * loads and stores; *avg_ms = mean launch time from events on `stream`.  Synchronous.  No
+ * counterpart in the reference (a measurement hook: the bandwidth a one-pass read-then-write
+ * kernel on these planes reaches). */
+int pvd_bench_plane_copy(const float* in, float* out, int32_t np, int64_t cap, int64_t n, int32_t reps,
+                         void* stream, float* avg_ms);
 
 /* One step with the caller's draws (device, float64): displacements (n, M, d) as
  * np.random.normal(0, sigma, (n, d, M)).transpose(0, 2, 1) (pyvibdmc.py:544-546),

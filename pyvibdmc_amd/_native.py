@@ -66,6 +66,8 @@ _SIGS = {
     'pvd_sim_start_external': (C.c_int, [_P, _P, _P]),
     'pvd_sim_run': (C.c_int, [_P, C.c_int32, _P]),
     'pvd_sim_run_timed': (C.c_int, [_P, C.c_int32, _P, C.POINTER(C.c_float), C.POINTER(C.c_int32)]),
+    'pvd_bench_plane_copy': (C.c_int, [_P, _P, C.c_int32, C.c_int64, C.c_int64, C.c_int32, _P,
+                                       C.POINTER(C.c_float)]),
     'pvd_sim_step_injected': (C.c_int, [_P, _P, _P, _P, _P]),
     'pvd_sim_move': (C.c_int, [_P, _P, _P, _P]),
     'pvd_sim_branch_external': (C.c_int, [_P, _P, _P, _P]),

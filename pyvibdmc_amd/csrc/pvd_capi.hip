@@ -244,6 +244,22 @@ void free_sim(pvd_sim* s) {
   delete s;
 }
 
+// Streaming copy of np SoA planes (component p of walker i at planes[p*cap + i]): each
+// thread moves 4 consecutive walkers of every plane with one 16-byte load and store (the
+// tail walkers one by one).  The read-then-write ceiling of the walker layout.
+__global__ __launch_bounds__(256) void k_plane_copy(const float* __restrict__ in, float* __restrict__ out, int np,
+                                                    long long cap, long long n) {
+  const long long i = ((long long)blockIdx.x * 256 + threadIdx.x) * 4;
+  if (i >= n) return;
+  if (i + 3 < n) {
+    for (int p = 0; p < np; ++p)
+      *(float4*)(out + p * cap + i) = *(const float4*)(in + p * cap + i);
+  } else {
+    for (int p = 0; p < np; ++p)
+      for (long long k = i; k < n; ++k) out[p * cap + k] = in[p * cap + k];
+  }
+}
+
 }  // namespace
 
 // ===================================================================== C entry points
@@ -470,6 +486,36 @@ int pvd_sim_run(pvd_sim* s, int32_t n_steps, void* stream) {
   if (n_steps < 0) return fail(E_BAD, "negative step count");
   return s->prec == PVD_FP64 ? run_steps<double>(s, n_steps, nullptr, nullptr, nullptr, (hipStream_t)stream)
                              : run_steps<float>(s, n_steps, nullptr, nullptr, nullptr, (hipStream_t)stream);
+}
+
+int pvd_bench_plane_copy(const float* in, float* out, int32_t np, int64_t cap, int64_t n, int32_t reps, void* stream,
+                         float* avg_ms) {
+  if (!in || !out || !avg_ms) return fail(E_BAD, "null argument");
+  if (np < 1 || n < 1 || cap < n || (cap & 3) || reps < 1) return fail(E_BAD, "bad plane shape");
+  if ((((uintptr_t)in) | ((uintptr_t)out)) & 15) return fail(E_BAD, "planes must be 16-byte aligned");
+  hipStream_t st = (hipStream_t)stream;
+  const unsigned grid = (unsigned)((n + 1023) / 1024);
+  hipEvent_t a = nullptr, b = nullptr;
+  HIPCHK(hipEventCreate(&a));
+  if (hipEventCreate(&b) != hipSuccess) {
+    (void)hipEventDestroy(a);
+    return fail(E_HIP, "hipEventCreate failed");
+  }
+  hipLaunchKernelGGL(k_plane_copy, dim3(grid), dim3(256), 0, st, in, out, (int)np, (long long)cap, (long long)n);
+  hipError_t e = hipEventRecord(a, st);
+  for (int r = 0; r < reps && e == hipSuccess; ++r) {
+    hipLaunchKernelGGL(k_plane_copy, dim3(grid), dim3(256), 0, st, in, out, (int)np, (long long)cap, (long long)n);
+    e = hipGetLastError();
+  }
+  if (e == hipSuccess) e = hipEventRecord(b, st);
+  if (e == hipSuccess) e = hipEventSynchronize(b);
+  float ms = 0.f;
+  if (e == hipSuccess) e = hipEventElapsedTime(&ms, a, b);
+  (void)hipEventDestroy(a);
+  (void)hipEventDestroy(b);
+  if (e != hipSuccess) return fail(E_HIP, std::string("plane copy: ") + hipGetErrorString(e));
+  *avg_ms = ms / (float)reps;
+  return 0;
 }
 
 int pvd_sim_run_timed(pvd_sim* s, int32_t n_steps, void* stream, float* avg_ms, int32_t* n_kernels) {

@@ -203,6 +203,43 @@ def test_is_fp32_fast_trial_path_matches_fp64_zpe():
     assert 4670 < z['fp64'] < 4690, z
 
 
+def test_c1_continuous_fp32_with_descendant_weighting_tracks_fp64():
+    """The continuous-weighting workload: triatomic surface, 100k walkers, dt = 5, 10 000 steps,
+    lower weight threshold 0.01, descendant weighting every 500 steps over 50 steps, FP32
+    propagation against the FP64 run.  Both give the same ZPE within 3 combined blocking
+    errors, the same number of descendant windows, and the same descendant-weighted mean O-H
+    distance (the psi^2 expectation the windows exist for) within 0.002 bohr."""
+    import pyvibdmc_amd as pv
+    WN = 4.556335281212229e-6
+    n, T = 100_000, 10_000
+    start = P.water_eq()[None] + np.random.RandomState(0).normal(0.0, 0.05, (n, 3, 3))
+    out = {}
+    for prec in ('fp32', 'fp64'):
+        sim = pv.DMC_Sim(num_walkers=n, num_timesteps=T, atoms=['H', 'H', 'O'], masses=[P.M_H, P.M_H, P.M_O],
+                         delta_t=5, potential=pv.MorseBendTriatomic(), start_structures=start,
+                         weighting='continuous', cont_wt_thresh=0.01, equil_steps=500, wfn_every=500,
+                         desc_wt_steps=50, chkpt_every=10 ** 9, precision=prec, seed=4, write_files=False)
+        sim.propagate()
+        v = sim._vref_vs_tau[T // 4:] / WN
+        se = 0.0
+        b = 1
+        while len(v) // b >= 16:
+            m = len(v) // b
+            se = max(se, v[:m * b].reshape(m, b).mean(axis=1).std(ddof=1) / np.sqrt(m))
+            b *= 2
+        roh = []
+        for _, rec in sim.wfns:
+            c, w = rec['coords'], rec['desc_wts']
+            assert np.all(w >= 0) and w.sum() > 0.5 * n
+            r = 0.5 * (np.linalg.norm(c[:, 0] - c[:, 2], axis=1) + np.linalg.norm(c[:, 1] - c[:, 2], axis=1))
+            roh.append(np.sum(w * r) / np.sum(w))
+        out[prec] = (v.mean(), se, len(sim.wfns), np.mean(roh))
+    (z32, s32, k32, r32), (z64, s64, k64, r64) = out['fp32'], out['fp64']
+    assert k32 == k64 >= 18, out
+    assert abs(z32 - z64) < 3 * np.hypot(s32, s64), out
+    assert abs(r32 - r64) < 2e-3 and 1.80 < r64 < 1.90, out
+
+
 def test_is_fp32_cluster_matches_fp64_zpe():
     """Six-atom cluster with the per-monomer trial: the FP32 move (gradients + closed-form
     weighted Laplacian, runtime bond/angle lists) and the FP64 move (per-component
