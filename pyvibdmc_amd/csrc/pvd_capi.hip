@@ -235,6 +235,8 @@ void free_sim(pvd_sim* s) {
   if (!s) return;
   for (int i = 0; i < 2; ++i)
     if (s->graph[i]) (void)hipGraphExecDestroy(s->graph[i]);
+  drop_surrogate_graphs(s);
+  if (s->cap_stream) (void)hipStreamDestroy(s->cap_stream);
   void* ptrs[] = {s->coords[0], s->coords[1], s->who[0], s->who[1], s->w, s->E, s->ke, s->kmask,
                   s->ctrl, s->flags, s->gflags, s->rec, s->drec, s->accs, s->tr_vref, s->tr_pop, s->tr_dteff, s->tr_births,
                   s->tr_deaths, s->pend, s->fin, s->klist, s->killtag, s->rstage, s->cand, s->asg, s->cpy, s->stage, s->parent, s->parent_w, s->desc,
@@ -610,6 +612,7 @@ int pvd_sim_set_surrogate(pvd_sim* s, const pvd_mlp* net, const pvd_pes_desc* fa
     s->jcap = (int)std::min<long long>(1 << 20, s->tmax / judge_every + 2);
     HIPCHK(hipMalloc((void**)&s->jlog, sizeof(double) * 4 * s->jcap));
   }
+  drop_surrogate_graphs(s);  // they carry the previous description's scalars
   s->surr = *net;
   s->surr_fb = *fallback;
   s->judge_every = judge_every;

@@ -89,6 +89,10 @@ struct pvd_sim {
   int slot_val[16];
   int n_slot;
   hipGraphExec_t graph[2];  // replayed step graphs, without / with descendant tracking
+  // surrogate steps replayed as graphs of 2^k steps (k = 1..6), without / with descendant
+  // tracking; captured on cap_stream and dropped whenever the network description changes
+  hipGraphExec_t sgraph[2][7];
+  hipStream_t cap_stream;
   // surrogate potential (external-potential simulations only)
   int surr_on;
   pvd_mlp surr;             // network description; its device parameters belong to the caller
@@ -105,6 +109,15 @@ inline size_t esize(const pvd_sim* s) { return s->prec == PVD_FP64 ? 8 : 4; }
 
 // buffer holding the current walkers (discrete: by step parity; continuous: in place)
 inline int cur_buf(const pvd_sim* s) { return s->cont ? 0 : (int)(s->next_step & 1); }
+
+inline void drop_surrogate_graphs(pvd_sim* s) {
+  for (int a = 0; a < 2; ++a)
+    for (int k = 0; k < 7; ++k)
+      if (s->sgraph[a][k]) {
+        (void)hipGraphExecDestroy(s->sgraph[a][k]);
+        s->sgraph[a][k] = nullptr;
+      }
+}
 
 template <typename R>
 StepArgs<R> make_args(pvd_sim* s) {

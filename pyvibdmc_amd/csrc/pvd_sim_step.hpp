@@ -213,17 +213,71 @@ int surrogate_step(pvd_sim* s, hipStream_t st) {
   return do_branch_ext<R>(s, s->vsur, nullptr, st);
 }
 
+// A graph of g = 2^lg surrogate steps from an even step, captured on the handle's own stream:
+// every launch of a step is parameter-identical to the one two steps later (step-dependent
+// state lives in the device control block), so the graph replays at any even step.  The judge
+// runs outside graphs, on its own steps.
+template <typename R>
+int build_surrogate_graph(pvd_sim* s, int lg, hipGraphExec_t* out) {
+  if (!s->cap_stream) HIPCHK(hipStreamCreateWithFlags(&s->cap_stream, hipStreamNonBlocking));
+  const long long saved = s->next_step, saved_e = s->epoch_next;
+  const int saved_je = s->judge_every;
+  s->judge_every = 0;
+  HIPCHK(hipStreamBeginCapture(s->cap_stream, hipStreamCaptureModeRelaxed));
+  int rc = 0;
+  for (int k = 0; k < (1 << lg) && rc == 0; ++k) rc = surrogate_step<R>(s, s->cap_stream);
+  hipGraph_t g = nullptr;
+  const hipError_t e = hipStreamEndCapture(s->cap_stream, &g);
+  s->next_step = saved;
+  s->epoch_next = saved_e;
+  s->judge_every = saved_je;
+  if (rc == 0 && e != hipSuccess) rc = fail(E_HIP, std::string("surrogate graph capture: ") + hipGetErrorString(e));
+  if (rc == 0) {
+    const hipError_t ei = hipGraphInstantiate(out, g, nullptr, nullptr, 0);
+    if (ei != hipSuccess) rc = fail(E_HIP, std::string("hipGraphInstantiate: ") + hipGetErrorString(ei));
+  }
+  if (g) (void)hipGraphDestroy(g);
+  return rc;
+}
+
+template <typename R>
+int run_surrogate_steps(pvd_sim* s, int n_steps, hipStream_t st) {
+  int k = 0;
+  while (k < n_steps) {
+    // steps before the next judge step (which runs eagerly)
+    long long free_steps = n_steps - k;
+    if (s->judge_every > 0 && s->jn < s->jcap) {
+      const long long t = s->next_step, je = s->judge_every;
+      free_steps = std::min(free_steps, ((t + je) / je) * je - 1 - t);
+    }
+    if (s->use_graphs && s->cfg.branch_every <= 1 && !(s->next_step & 1) && free_steps >= 2) {
+      int lg = 6;
+      while ((1ll << lg) > free_steps) --lg;
+      hipGraphExec_t& ge = s->sgraph[s->who_on ? 1 : 0][lg];
+      if (!ge) {
+        const int rc = build_surrogate_graph<R>(s, lg, &ge);
+        if (rc) return rc;
+      }
+      HIPCHK(hipGraphLaunch(ge, st));
+      s->next_step += 1ll << lg;
+      s->epoch_next += 1ll << lg;
+      k += 1 << lg;
+      continue;
+    }
+    const int rc = surrogate_step<R>(s, st);
+    if (rc) return rc;
+    ++k;
+  }
+  return 0;
+}
+
 template <typename R>
 int run_steps(pvd_sim* s, int n_steps, const double* disp, const double* um, const double* ub,
               hipStream_t st) {
   if (s->htrial) return fail(E_BAD, "a caller-evaluated trial runs the split step (is_propose / is_accept)");
   if (s->surr_on) {
     if (disp) return fail(E_BAD, "injected draws with a surrogate potential: use move / branch_external");
-    for (int k = 0; k < n_steps; ++k) {
-      const int rc = surrogate_step<R>(s, st);
-      if (rc) return rc;
-    }
-    return 0;
+    return run_surrogate_steps<R>(s, n_steps, st);
   }
   int k = 0;
   const bool graphs = !disp && s->use_graphs && (s->cfg.branch_every <= 1) && n_steps >= PVD_GRAPH_STEPS + 1;

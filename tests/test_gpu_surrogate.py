@@ -282,3 +282,30 @@ def test_persistent_inference_over_many_chunks(hidden):
     ref = y * m.desc.ystd + m.desc.ymean + O.quad_terms(normalised_features(m, xs)) @ np.array(m.desc.quad[:10])
     err = np.linalg.norm(v_one - ref) / np.linalg.norm(ref - ref.mean())
     assert err < 2e-3, err
+
+
+def test_surrogate_steps_as_graphs_match_eager_launches(monkeypatch):
+    """300 importance-sampled surrogate steps (20 000 walkers, judge every 100 steps): replayed
+    as captured graphs of 2 to 64 steps between the judge steps, they give the same vref,
+    populations, judge records and final walkers, bit for bit, as one launch at a time."""
+    import pyvibdmc_amd as pv
+    from pyvibdmc_amd.impsamp import water_trial
+    from pyvibdmc_amd.surrogate import SurrogatePotential
+    m = pv.SurrogateModel(WATER, hidden=128, seed=3)
+    x = water_walkers(20000, 4)
+    m.set_normalisation(x, P.morse_bend(x), range_margin=0.2)
+    out = {}
+    for graphs in ('1', '0'):
+        monkeypatch.setenv('PVD_GRAPHS', graphs)
+        pot = SurrogatePotential(m, pv.MorseBendTriatomic(), judge_every=100, judge_stride=7)
+        sim = pv.DMC_Sim(num_walkers=20000, num_timesteps=300, atoms=WATER, masses=[P.M_H, P.M_H, P.M_O], delta_t=5,
+                         potential=pot, start_structures=water_walkers(20000, 5, 0.05), imp_samp=water_trial(),
+                         precision='fp32', seed=21, write_files=False, equil_steps=10 ** 9, wfn_every=10 ** 9,
+                         chkpt_every=10 ** 9)
+        monkeypatch.delenv('PVD_GRAPHS')
+        sim.propagate()
+        coords, _ = sim._get_walkers()
+        out[graphs] = (sim._vref_vs_tau.copy(), sim._pop_vs_tau.copy(), np.array(pot.judge_log), coords)
+    for a, b in zip(out['1'], out['0']):
+        np.testing.assert_array_equal(a, b)
+    assert len(out['1'][2]) == 3
