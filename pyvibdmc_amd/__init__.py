@@ -20,5 +20,6 @@ from ._native import DMCError, load as load_library
 from .analysis import SimInfo
 from .initial_conditions import HarmonicAnalysis, InitialConditioner, MolFiniteDifference
 from .hdf5 import save_h5
+from . import torch_ops
 
 __version__ = '0.1.0'
