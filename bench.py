@@ -352,6 +352,57 @@ def time_to_accuracy_study(model, oracle, n=100_000, tol=1.0, sigma=0.3):
     return out
 
 
+def ps_variant(n, steps, epochs, zpe_steps=2000):
+    """The Partridge-Schwenke water surface (the reference's Fortran potential as a device
+    kernel): importance-sampled DMC at n walkers with the analytic surface and with a
+    3 -> 128^3 -> 1 surrogate trained on 2M geometries an analytic run visits (200 epochs),
+    the analytic surface as its fallback and judge.  Throughput of both forms and their ZPEs
+    over zpe_steps steps (same seed and start)."""
+    import torch
+    import pyvibdmc_amd as pv
+    from pyvibdmc_amd.impsamp import water_trial
+    from pyvibdmc_amd.surrogate import SurrogatePotential
+
+    def sim(nw, T, pot, seed, train_every=None):
+        start = pv.PartridgeSchwenkeWater.equilibrium()[None] + np.random.RandomState(seed).normal(0, 0.05, (nw, 3, 3))
+        return pv.DMC_Sim(num_walkers=nw, num_timesteps=T, atoms=['H', 'H', 'O'], delta_t=5, potential=pot,
+                          start_structures=start, imp_samp=water_trial(), precision='fp32', seed=seed,
+                          write_files=False, equil_steps=10 ** 9, wfn_every=10 ** 9, chkpt_every=10 ** 9,
+                          DEBUG_save_training_every=train_every)
+    s = sim(1_000_000, 601, pv.PartridgeSchwenkeWater(), 11, train_every=300)
+    s.propagate()
+    data = [(c, v) for st, c, v in s.training if st > 0]
+    coords = np.concatenate([c for c, _ in data])
+    pots = np.concatenate([v for _, v in data])
+    del s, data
+    m = pv.SurrogateModel(['H', 'H', 'O'], hidden=128)
+    m.fit(coords, pots, epochs=epochs, batch=8192, lr=1e-3)
+    te = coords[::197]
+    err = (m.energy(te) - pv.PartridgeSchwenkeWater().getpot(te)) / WN
+    out = {"config": f"Partridge-Schwenke water, {n} walkers, IS (Gaussian-product psi_T), dt=5, FP32",
+           "training": {"geometries": int(len(coords)), "epochs": epochs, "train_seconds": m.train_seconds,
+                        "rms_err_cm": float(np.sqrt(np.mean(err ** 2))), "max_err_cm": float(np.abs(err).max())}}
+    del coords, pots
+    for name, mk in (('analytic', lambda: pv.PartridgeSchwenkeWater()),
+                     ('surrogate', lambda: SurrogatePotential(m, pv.PartridgeSchwenkeWater(), judge_every=100))):
+        sm = sim(n, 10 + steps + 2, mk(), 7)
+        ws, wall, _, _, _ = run_timed(sm, 10, steps, 0)
+        del sm
+        pot = mk()
+        sz = sim(n, zpe_steps, pot, 7)
+        sz.propagate()
+        z, se = zpe_of(sz._vref_vs_tau)
+        out[name] = {"value": ws / wall, "unit": "walker-steps/s", "ms_per_step": wall * 1e3 / steps, "zpe_cm": z,
+                     "zpe_se_cm": se}
+        if name == 'surrogate':
+            out[name]["judge_rms_cm"] = [r[1] / WN for r in pot.judge_log][-3:]
+        log(f'[bench] PS {name}: {ws / wall:.4g} walker-steps/s, ZPE {z:.2f} +- {se:.2f}')
+        del sz
+        torch.cuda.empty_cache()
+    out["zpe_surrogate_minus_analytic_cm"] = out['surrogate']['zpe_cm'] - out['analytic']['zpe_cm']
+    return out
+
+
 CLUSTER_EL = ['O', 'H', 'H', 'O', 'H', 'H']
 # algorithmic FLOP of the 15 -> 256 -> 256 -> 256 -> 1 network per walker
 MLP256_FLOP_PER_WALKER = 2 * (15 * 256 + 256 * 256 + 256 * 256 + 256)
@@ -651,6 +702,10 @@ def main():
         torch.cuda.empty_cache()
     else:
         model = None
+
+    if not args.no_variants and not args.no_surrogate:
+        log('[bench] Partridge-Schwenke surface: analytic and surrogate')
+        variants['ps_surface'] = ps_variant(n, K, args.epochs)
 
     zpe = tta = None
     if not args.no_variants and args.zpe_steps > 0:

@@ -309,3 +309,40 @@ def test_surrogate_steps_as_graphs_match_eager_launches(monkeypatch):
     for a, b in zip(out['1'], out['0']):
         np.testing.assert_array_equal(a, b)
     assert len(out['1'][2]) == 3
+
+
+@pytest.mark.timeout(300)
+def test_surrogate_on_the_partridge_schwenke_surface():
+    """The learned part on the reference's own Fortran surface (the Partridge-Schwenke device
+    potential as labels, fallback and judge): 3 -> 128^3 -> 1 trained 200 epochs on 2M
+    geometries an importance-sampled run on that surface visits; 1M-walker IS DMC with it
+    gives the analytic surface's ZPE within 1 cm^-1, the judge's error stays below 10 cm^-1."""
+    import pyvibdmc_amd as pv
+    from pyvibdmc_amd.impsamp import water_trial
+    from pyvibdmc_amd.surrogate import SurrogatePotential
+    WN = 4.556335281212229e-6
+    T = 2000
+
+    def sim(n, steps, pot, seed, train_every=None):
+        start = P.ps_eq()[None] + np.random.RandomState(seed).normal(0, 0.05, (n, 3, 3))
+        return pv.DMC_Sim(num_walkers=n, num_timesteps=steps, atoms=WATER, delta_t=5, potential=pot,
+                          start_structures=start, imp_samp=water_trial(), precision='fp32', seed=seed,
+                          write_files=False, equil_steps=10 ** 9, wfn_every=10 ** 9, chkpt_every=10 ** 9,
+                          DEBUG_save_training_every=train_every)
+    s = sim(1_000_000, 601, pv.PartridgeSchwenkeWater(), 11, train_every=300)
+    s.propagate()
+    data = [(c, v) for st, c, v in s.training if st > 0]
+    coords = np.concatenate([c for c, _ in data])
+    pots = np.concatenate([v for _, v in data])
+    m = pv.SurrogateModel(WATER, hidden=128)
+    m.fit(coords, pots, epochs=200, batch=8192, lr=1e-3)
+    z = {}
+    for name, pot in (('analytic', pv.PartridgeSchwenkeWater()),
+                      ('surrogate', SurrogatePotential(m, pv.PartridgeSchwenkeWater(), judge_every=100))):
+        r = sim(1_000_000, T, pot, 7)
+        r.propagate()
+        z[name] = np.mean(r._vref_vs_tau[T // 4:]) / WN
+        if name == 'surrogate':
+            assert max(rec[1] for rec in pot.judge_log) / WN < 10.0
+    assert 4600 < z['analytic'] < 4680, z
+    assert abs(z['surrogate'] - z['analytic']) < 1.0, z
