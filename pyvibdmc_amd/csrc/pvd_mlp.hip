@@ -216,11 +216,15 @@ PVD_DEV bf16x8 input_chunk(const MlpNet& M, const float* f, int g, bool valid) {
 template <int NT, int MT, int KC>
 PVD_DEV void mfma_layer(const bf16x8* __restrict__ Wl, const bf16x8 (&bin)[NT][KC], f32x4 (&acc)[NT][MT], int lane) {
 #pragma unroll
-  for (int m = 0; m < MT; ++m) {
+  for (int m = 0; m < MT; ++m)
 #pragma unroll
     for (int t = 0; t < NT; ++t) acc[t][m] = f32x4{0.f, 0.f, 0.f, 0.f};
+  // input chunk outer, output tile inner: consecutive MFMAs write different accumulators
+  // (each accumulator still sums its chunks in order c = 0, 1, ...)
 #pragma unroll
-    for (int c = 0; c < KC; ++c) {
+  for (int c = 0; c < KC; ++c) {
+#pragma unroll
+    for (int m = 0; m < MT; ++m) {
       const bf16x8 a = Wl[(m * KC + c) * 64 + lane];
 #pragma unroll
       for (int t = 0; t < NT; ++t) acc[t][m] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, bin[t][c], acc[t][m], 0, 0, 0);
@@ -910,15 +914,17 @@ __global__ __launch_bounds__(64 * WV, 1) void k_mlp_energy_st(MlpNet M, const R*
       for (int q = threadIdx.x; q < MT * KC * 64; q += NTH) LW[q] = src[q];
       __syncthreads();
       const float* bl = lb + layer * H;
+      // input chunk outer, output tile inner: consecutive MFMAs write different
+      // accumulators, so none waits on the previous one's result
 #pragma unroll
-      for (int m = 0; m < MT; ++m) {
-        const f32x4 bias = *(const f32x4*)(bl + 16 * m + 4 * g);
+      for (int c = 0; c < KC; ++c) {
 #pragma unroll
-        for (int c = 0; c < KC; ++c) {
+        for (int m = 0; m < MT; ++m) {
           const bf16x8 a = LW[(m * KC + c) * 64 + lane];
 #pragma unroll
           for (int t = 0; t < NTS; ++t)
-            acc[t][m] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, bin[t][c], c == 0 ? bias : acc[t][m], 0, 0, 0);
+            acc[t][m] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(
+                a, bin[t][c], c == 0 ? *(const f32x4*)(bl + 16 * m + 4 * g) : acc[t][m], 0, 0, 0);
         }
       }
       if (layer == 1) {
