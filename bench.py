@@ -399,6 +399,15 @@ def ps_variant(n, steps, epochs, zpe_steps=2000):
         log(f'[bench] PS {name}: {ws / wall:.4g} walker-steps/s, ZPE {z:.2f} +- {se:.2f}')
         del sz
         torch.cuda.empty_cache()
+    # plain DMC (no trial function) on the analytic surface, for the time-to-accuracy comparison
+    start = pv.PartridgeSchwenkeWater.equilibrium()[None] + np.random.RandomState(7).normal(0, 0.05, (n, 3, 3))
+    sp = pv.DMC_Sim(num_walkers=n, num_timesteps=10 + steps + 2, atoms=['H', 'H', 'O'], delta_t=5,
+                    potential=pv.PartridgeSchwenkeWater(), start_structures=start, precision='fp32', seed=7,
+                    write_files=False, equil_steps=10 ** 9, wfn_every=10 ** 9, chkpt_every=10 ** 9)
+    ws, wall, _, _, _ = run_timed(sp, 10, steps, 0)
+    out['plain_analytic'] = {"value": ws / wall, "unit": "walker-steps/s", "ms_per_step": wall * 1e3 / steps}
+    log(f'[bench] PS plain analytic: {ws / wall:.4g} walker-steps/s')
+    del sp
     out["zpe_surrogate_minus_analytic_cm"] = out['surrogate']['zpe_cm'] - out['analytic']['zpe_cm']
     return out
 
@@ -566,6 +575,7 @@ def main():
     ap.add_argument('--cluster-only', action='store_true',
                     help='only the six-atom cluster workload (16M walkers by default, --walkers to change)')
     ap.add_argument('--cluster-zpe-steps', type=int, default=2000)
+    ap.add_argument('--ps-only', action='store_true', help='only the Partridge-Schwenke surface variant')
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU: the DMC walkers of this workload live on a single MI355X "
@@ -579,6 +589,9 @@ def main():
         return
     if args.continuous_only:
         print(json.dumps({"continuous": continuous_variant()}), flush=True)
+        return
+    if args.ps_only:
+        print(json.dumps({"ps_surface": ps_variant(args.walkers, args.steps, args.epochs)}), flush=True)
         return
     if args.cluster_only:
         n_c = args.walkers if args.walkers != 1_000_000 else 16_000_000
