@@ -83,7 +83,11 @@ struct Cand {
   int slot;
   int pad;
 };
-constexpr int CAND_CAP = 4096;   // candidates the replay sorts
+constexpr int CAND_CAP = 4096;   // candidates the replay sorts in LDS
+constexpr int CAND_G = 32768;    // candidates the global-memory replay sorts (more kills per step)
+// scratch of the global-memory replay: keys and bucketed keys (8 B each), order and bucketed
+// order (4 B each), family starts (4 B), split depths (1 B) per candidate
+constexpr size_t RSG_BYTES = (size_t)CAND_G * (8 + 8 + 4 + 4 + 4 + 1);
 constexpr int LIST_CAP = 32768;  // entries per candidate list and step
 constexpr int LIST_SUB = 8;      // sub-lists per list (spreads the appends' counters)
 constexpr int SUB_CAP = LIST_CAP / LIST_SUB;
@@ -131,6 +135,7 @@ struct StepArgs {
   unsigned long long* killtag;  // per slot: (step << 32) | kill index, valid for this step only
   double* rstage;          // staged source walkers of the split copies
   Cand* cand;              // candidate walkers of the split replay (unordered)
+  unsigned char* rsg;      // scratch of the global-memory candidate replay (RSG_BYTES)
   int* asg;                // per kill index: the candidate (sorted position) it copies
   int* cpy;                // per kill index: (kill slot, source slot) for the copy kernel
   PesParams pes;
@@ -1299,8 +1304,9 @@ PVD_DEV double th_pick(const int* hist, int target, double thresh, int* s_i, int
 }
 
 // List A size the threshold aims at: half as many again as the kills (at least 1024), so
-// that the replay needs no cut inside the list.
-PVD_DEV int th_target(int K) { return min(max(K + K / 2, 1024), CAND_CAP - CAND_CAP / 8); }
+// that the replay needs no cut inside the list (the LDS replay cuts lists longer than it
+// sorts; with more than CAND_CAP / 1.25 kills the global-memory replay takes them).
+PVD_DEV int th_target(int K) { return min(max(K + K / 2, 1024), CAND_G - CAND_G / 8); }
 
 // Closed-form replay of the sequential split loop (pyvibdmc.py:344-356) from the
 // candidate list: the walkers of weight >= T0, where T0 >= the lower threshold.
@@ -1641,6 +1647,249 @@ PVD_DEV int replay_candidates(const StepArgs<R>& A, int K, int nlists, const int
   return RC_DONE;
 }
 
+// The same closed-form replay when the kills outgrow the LDS sort (1.25 K > CAND_CAP): up to
+// CAND_G candidates, sorted and scanned through an L2-resident scratch (A.rsg) instead of LDS
+// and registers.  Keys, order, family starts and depths live in the scratch, the bucket
+// counts of the counting sort in LDS (`bcnt`, RC_BUCKETS ints).  Each octave's exclusive scan
+// of family sizes runs over chunks of RP_NT families with a carry, and every family writes the
+// kill indices of its own run.  It declines where the LDS form declines, and also when one
+// mantissa bucket holds more than 64 candidates.
+template <typename R>
+PVD_DEV int replay_candidates_g(const StepArgs<R>& A, int K, int nlists, const int* subn, double T0, int* bcnt,
+                                int* s_i, int* s_sh, int* hist, double& cut, bool& refined) {
+  constexpr int NT = RP_NT;
+  constexpr unsigned long long MANT = (1ull << 52) - 1;
+  constexpr int NS = 2 * LIST_SUB;
+  const int tid = threadIdx.x;
+  int pre[NS + 1];
+  pre[0] = 0;
+#pragma unroll
+  for (int c = 0; c < NS; ++c) {
+    const int k = c < nlists * LIST_SUB ? subn[c] : 0;
+    if (k > SUB_CAP) return RC_MANY;  // entries were lost
+    pre[c + 1] = pre[c] + k;
+  }
+  const int nl = pre[NS];
+  if (nl <= 0) return RC_FEW;
+  if (K > KILL_CAP) return RC_OTHER;
+  if (!(T0 >= A.thresh_lo) || !(T0 > 0.0)) return RC_OTHER;
+  auto phys = [&](int q) -> int {
+    int off = 0;
+#pragma unroll
+    for (int u = 0; u < NS; ++u)
+      if (q >= pre[u]) off = u * SUB_CAP - pre[u];
+    return q + off;
+  };
+  const long long tb = th_base(A.thresh_lo);
+  unsigned long long* key = reinterpret_cast<unsigned long long*>(A.rsg);
+  unsigned long long* bkey = key + CAND_G;
+  int* ord = reinterpret_cast<int*>(bkey + CAND_G);
+  int* bord = ord + CAND_G;
+  int* start = bord + CAND_G;
+  unsigned char* dep = reinterpret_cast<unsigned char*>(start + CAND_G);
+  // key -> exponent and mantissa of the candidate's weight (the key is built below)
+  auto kexp = [](unsigned long long k) { return (int)(0x7FFull - (k & 0x7FFull)) - 1023; };
+  auto kmant = [](unsigned long long k) { return MANT - (k >> 11); };
+  // 1. the cut: T0 when the list fits the sort, else the lowest histogram edge with at most
+  //    about 1.25 K entries at or above it
+  double T = T0;
+  refined = nl > CAND_G;
+  if (refined) {
+    for (int b = tid; b < TH_BINS; b += NT) hist[b] = 0;
+    __syncthreads();
+    for (int q = tid; q < nl; q += NT) atomicAdd(&hist[th_bin(A.cand[phys(q)].w, tb)], 1);
+    __syncthreads();
+    const int lim = min(CAND_G, max(K + K / 4, 1024));
+    const int over = th_cross(hist, lim, true, s_i, s_sh);
+    if (over < 0 || over >= TH_BINS - 1) return RC_MANY;
+    T = fmax(T0, th_edge(A.thresh_lo, over + 1));
+  }
+  // the walkers >= T, in any order (wave ballots onto one counter)
+  if (tid == 0) s_sh[3] = 0;
+  __syncthreads();
+  for (int q0 = 0; q0 < nl; q0 += NT) {
+    const int q = q0 + tid;
+    const int pq = q < nl ? phys(q) : 0;
+    double w = 0.0;
+    bool in = false;
+    if (q < nl) {
+      w = A.cand[pq].w;
+      in = w >= T;
+    }
+    const unsigned long long bal = __ballot(in);
+    if (bal) {
+      const int lead = __ffsll((long long)bal) - 1;
+      int base = 0;
+      if (lane_id() == lead) base = atomicAdd(&s_sh[3], __popcll(bal));
+      base = __shfl(base, lead, PVD_WAVE);
+      const int o = base + __popcll(bal & ((1ull << lane_id()) - 1ull));
+      if (in && o < CAND_G) {
+        const unsigned long long wb = (unsigned long long)__double_as_longlong(w);
+        key[o] = ((MANT - (wb & MANT)) << 11) | (0x7FFull - ((wb >> 52) & 0x7FFull));
+        ord[o] = pq;
+      }
+    }
+  }
+  __syncthreads();
+  const int C = s_sh[3];
+  if (C > CAND_G) return RC_MANY;
+  if (C == 0) return RC_FEW;
+  // 2. candidates by decreasing mantissa (then exponent): counting sort on the top 12 bits of
+  //    the 63-bit key, then each entry ranked inside its bucket
+  constexpr int BSH = 63 - 12;
+  for (int b = tid; b < RC_BUCKETS; b += NT) bcnt[b] = 0;
+  __syncthreads();
+  for (int q = tid; q < C; q += NT) atomicAdd(&bcnt[(int)(key[q] >> BSH)], 1);
+  __syncthreads();
+  int bmax;
+  {
+    int c4[4], loc = 0, mx = 0;
+#pragma unroll
+    for (int u = 0; u < 4; ++u) {
+      c4[u] = bcnt[tid * 4 + u];
+      loc += c4[u];
+      mx = max(mx, c4[u]);
+    }
+    int tot;
+    const int ex = block_excl_scan<NT>(loc, s_i, tot);
+    int st = ex;
+#pragma unroll
+    for (int u = 0; u < 4; ++u) {
+      bcnt[tid * 4 + u] = st;  // bucket starts
+      st += c4[u];
+    }
+    bmax = block_max<NT>(mx, s_i);
+  }
+  if (bmax > 64) return RC_OTHER;
+  for (int q = tid; q < C; q += NT) {
+    const int d = atomicAdd(&bcnt[(int)(key[q] >> BSH)], 1);  // bcnt ends as the bucket ends
+    bkey[d] = key[q];
+    bord[d] = ord[q];
+  }
+  __syncthreads();
+  for (int q = tid; q < C; q += NT) {
+    const unsigned long long kq = bkey[q];
+    const int bk = (int)(kq >> BSH);
+    const int bs = bk > 0 ? bcnt[bk - 1] : 0, be = bcnt[bk];
+    int rank = 0;
+    for (int i = bs; i < be; ++i) {
+      const unsigned long long ki = bkey[i];
+      rank += (ki < kq || (ki == kq && i < q)) ? 1 : 0;
+    }
+    key[bs + rank] = kq;
+    ord[bs + rank] = bord[q];
+  }
+  __syncthreads();
+  // 3. the top octave, and the highest octave in which two equal mantissas meet
+  int etop = -0x40000000, tie_e = -0x40000000;
+  for (int j = tid; j < C; j += NT) {
+    const unsigned long long kj = key[j];
+    etop = max(etop, kexp(kj));
+    if (j + 1 < C && (key[j + 1] >> 11) == (kj >> 11)) tie_e = max(tie_e, min(kexp(kj), kexp(key[j + 1])));
+  }
+  etop = block_max<NT>(etop, s_i);
+  tie_e = block_max<NT>(tie_e, s_i);
+  const int eT = dexp(T);
+  const unsigned long long mT = (unsigned long long)__double_as_longlong(T) & MANT;
+  if (etop - eT > 40) return RC_OTHER;
+  // 4. octaves from the top: each family's run of kill indices, until K are used; in the
+  //    octave of T only the entries of value >= T count
+  int base = 0, Es = 0;
+  bool found = false;
+  for (int E = etop; E >= eT; --E) {
+    int carry = base;
+    for (int q0 = 0; q0 < C; q0 += NT) {
+      const int j = q0 + tid;
+      int v = 0;
+      if (j < C) {
+        const unsigned long long kj = key[j];
+        const int ej = kexp(kj);
+        if (ej >= E && (E > eT || kmant(kj) >= mT)) {
+          const int d = ej - E;
+          v = d >= 20 ? KILL_CAP + 1 : min(1 << d, KILL_CAP + 1);
+        }
+      }
+      int tot;
+      const int ex = block_excl_scan<NT>(v, s_i, tot);
+      const int st = carry + ex;
+      if (v > 0 && st < K) {
+        const int hi = min(st + v, K);
+        for (int t = st; t < hi; ++t) A.asg[t] = j;
+        if (st + v >= K) {
+          s_sh[0] = j;
+          s_sh[1] = st;
+        }
+      }
+      carry += tot;
+    }
+    __syncthreads();
+    if (carry >= K) {
+      Es = E;
+      found = true;
+      break;
+    }
+    base = carry;
+  }
+  if (!found) return RC_FEW;
+  if (tie_e >= Es) return RC_OTHER;
+  const int p = s_sh[0], startp = s_sh[1];
+  const int r = K - startp;
+  const Cand cp = A.cand[ord[p]];
+  const int Dp = dexp(cp.w) - Es;
+  const int m = 1 << Dp;
+  if (Dp > 13) return RC_OTHER;  // the family's slots must fit the scratch
+  for (int j = tid; j < C; j += NT) {
+    const int ej = kexp(key[j]);
+    dep[j] = (unsigned char)(ej < Es ? 0 : ej - Es + (j < p ? 1 : 0));
+  }
+  // 5. the slots of family p: the kill slots of its full levels, then its own
+  int* sp = start;
+  int cnt = 0;
+  for (int t0 = 0; t0 < startp; t0 += NT) {
+    const int t = t0 + tid;
+    const bool mem = t < startp && A.asg[t] == p;
+    int tot;
+    const int ex = block_excl_scan<NT>(mem ? 1 : 0, s_i, tot);
+    if (mem && cnt + ex < m) sp[cnt + ex] = A.klist[t];
+    cnt += tot;
+  }
+  if (cnt + 1 != m) return RC_OTHER;  // not reached: family p has 2^Dp entries
+  if (tid == 0) sp[m - 1] = cp.slot;
+  __syncthreads();
+  // 6. weights in place, and the list of copies (kill slot, source) for the copy kernel
+  for (int i = tid; i < m; i += NT) {
+    const int sl = sp[i];
+    int rank = 0;
+    for (int k = 0; k < m; ++k) rank += sp[k] < sl ? 1 : 0;
+    A.w[sl] = ldexp(cp.w, -(Dp + (rank < r ? 1 : 0)));
+  }
+  for (int t = tid; t < K; t += NT) {
+    const int j = A.asg[t];
+    const int ks = A.klist[t];
+    int o;
+    double wn;
+    if (j == p) {
+      o = cp.slot;
+      wn = ldexp(cp.w, -(Dp + 1));
+    } else {
+      const Cand c = A.cand[ord[j]];
+      o = c.slot;
+      wn = ldexp(c.w, -(int)dep[j]);
+    }
+    if (j != p || t >= startp) A.w[ks] = wn;
+    A.cpy[2 * t] = ks;
+    A.cpy[2 * t + 1] = o;
+  }
+  for (int j = tid; j < C; j += NT) {
+    if (j != p && dep[j] > 0) {
+      const Cand c = A.cand[ord[j]];
+      A.w[c.slot] = ldexp(c.w, -(int)dep[j]);
+    }
+  }
+  cut = ldexp(cp.w, -Dp);
+  return RC_DONE;
+}
+
 template <typename R>
 __global__ __launch_bounds__(RP_NT) void k_cont_replay(StepArgs<R> A) {
   __shared__ __align__(16) unsigned char s_buf[RP_LDS];
@@ -1816,15 +2065,22 @@ __global__ __launch_bounds__(RP_NT) void k_cont_replay(StepArgs<R> A) {
   {
     double cut = 0.0;
     bool refined = false;
-    rcode = replay_candidates(A, K, 1, s_subn, C->cand_t, s_buf, s_i, s_sh, s_hist, cut, refined);
+    // more kills than the LDS sort serves: the global-memory form
+    const bool big = K + K / 4 > CAND_CAP && A.rsg != nullptr;
+    auto replay = [&](int nlists, double T0) {
+      return big ? replay_candidates_g(A, K, nlists, s_subn, T0, reinterpret_cast<int*>(s_buf), s_i, s_sh, s_hist,
+                                       cut, refined)
+                 : replay_candidates(A, K, nlists, s_subn, T0, s_buf, s_i, s_sh, s_hist, cut, refined);
+    };
+    rcode = replay(1, C->cand_t);
     const bool miss = rcode != RC_DONE;
     if (rcode == RC_FEW && C->cand_t2 < C->cand_t) {
       __syncthreads();
-      rcode = replay_candidates(A, K, 2, s_subn, C->cand_t2, s_buf, s_i, s_sh, s_hist, cut, refined);
+      rcode = replay(2, C->cand_t2);
     }
     if (rcode == RC_DONE) {
       if (tid == 0) C->ncopy = K;
-      finish(s_a0n, s_a1n, cut, miss ? 6 : refined ? 5 : 1, 0.0, miss);
+      finish(s_a0n, s_a1n, cut, big ? (miss ? 8 : 7) : miss ? 6 : refined ? 5 : 1, 0.0, miss);
       return;
     }
   }

@@ -71,6 +71,7 @@ struct pvd_sim {
   unsigned long long* killtag;
   double* rstage;
   pvd::Cand* cand;
+  unsigned char* rsg;  // global-memory replay scratch (continuous weighting)
   int* asg;
   int* cpy;
   double* stage;
@@ -155,6 +156,7 @@ StepArgs<R> make_args(pvd_sim* s) {
   A.killtag = s->killtag;
   A.rstage = s->rstage;
   A.cand = s->cand;
+  A.rsg = s->rsg;
   A.asg = s->asg;
   A.cpy = s->cpy;
   std::memcpy(&A.pes, &s->cfg.pes, sizeof(PesParams));

@@ -126,6 +126,44 @@ def test_continuous_large_population_matches_oracle(n, steps):
     np.testing.assert_allclose(wts, orc.cont_wts, rtol=1e-12, atol=0)
 
 
+def test_continuous_replay_with_more_kills_than_the_lds_sort_matches_oracle():
+    """600k walkers, FP64, the reference's draws injected; start weights log-uniform over
+    [e^-2, e^2] (scaled to mean 1) with 20 % in a band just above the 0.01 threshold
+    (log-uniform over a tenth of an e-fold), so that 5 000 - 14 000 walkers cross it per step
+    -- more than the LDS replay serves (1.25 K > 4096).  The first step runs the interval replay (no list yet); every later step
+    with more than 3 277 kills takes the global-memory candidate replay; splits per step,
+    vref, weights and coordinates match the oracle's sequential loop."""
+    import pyvibdmc_amd as pv
+    from oracle import pes as P
+    from oracle.dmc import LegacyDraws, OracleDMC
+    n, steps = 600_000, 5
+    masses = [1837.15, 1837.15, 29156.95]
+    rs = np.random.RandomState(23)
+    start = P.water_eq()[None] + rs.normal(0, 0.05, (n, 3, 3))
+    w0 = np.exp(rs.uniform(-2.0, 2.0, n))  # the heavy end spread over octaves
+    band = rs.rand(n) < 0.2
+    w0[band] = 0.01 * np.exp(rs.uniform(0.0, 0.1, int(band.sum())))
+    w0[~band] *= (n - w0[band].sum()) / w0[~band].sum()  # mean weight 1: vref starts near v-bar
+    kw = dict(masses=masses, start_structures=start, num_walkers=n, num_timesteps=steps, delta_t=5,
+              equil_steps=10 ** 9, wfn_every=10 ** 9, weighting='continuous', cont_wt_thresh=0.01,
+              start_cont_wts=w0)
+    orc = OracleDMC(potential=P.morse_bend, draws=LegacyDraws(9), **kw)
+    orc.propagate()
+    sim = pv.DMC_Sim(atoms=['H', 'H', 'O'], potential=pv.MorseBendTriatomic(), precision='fp64',
+                     rng=LegacyDraws(9), write_files=False, chkpt_every=10 ** 9, **kw)
+    sim.propagate()
+    nb, how = sim.trajectory_counts(0, steps)
+    print('splits', nb, 'replay', how, flush=True)
+    np.testing.assert_array_equal(nb, orc.rec['n_branched'])
+    big = nb + nb // 4 > 4096
+    assert how[0] == 2 and big.sum() >= 2, (nb, how)
+    assert np.isin(how[1:][big[1:]], [7, 8]).all() and np.isin(how[1:][~big[1:]], [1, 5, 6]).all(), (nb, how)
+    np.testing.assert_allclose(sim._vref_vs_tau, orc._vref_vs_tau, rtol=0, atol=1e-10)
+    coords, wts = sim._get_walkers()
+    np.testing.assert_array_equal(coords, orc.coords)
+    np.testing.assert_allclose(wts, orc.cont_wts, rtol=1e-12, atol=0)
+
+
 def test_continuous_replay_with_a_cut_inside_the_list_matches_oracle():
     """A tenth of the walkers start in one dense cluster of near-equal heavy weights (5.0,
     spread 1e-6) and 3 % below the threshold.  After the first step (interval replay) the
