@@ -567,6 +567,8 @@ def main():
     ap.add_argument('--event-steps', type=int, default=200)
     ap.add_argument('--copy-only', action='store_true', help='only the 1 GiB copy (counter calibration)')
     ap.add_argument('--continuous-only', action='store_true', help='only the continuous-weighting workload')
+    ap.add_argument('--continuous-walkers', type=int, default=100_000)
+    ap.add_argument('--continuous-steps', type=int, default=10_000)
     ap.add_argument('--no-surrogate', action='store_true')
     ap.add_argument('--epochs', type=int, default=200)
     ap.add_argument('--train-geoms', type=int, default=2_000_000)
@@ -588,7 +590,8 @@ def main():
         print(json.dumps({"copy_GBs": copy_bandwidth()}), flush=True)
         return
     if args.continuous_only:
-        print(json.dumps({"continuous": continuous_variant()}), flush=True)
+        print(json.dumps({"continuous": continuous_variant(args.continuous_walkers, args.continuous_steps)}),
+              flush=True)
         return
     if args.ps_only:
         print(json.dumps({"ps_surface": ps_variant(args.walkers, args.steps, args.epochs)}), flush=True)

@@ -51,8 +51,8 @@ struct Ctrl {
   int ntd[2];
   // continuous split replay: the weight kernel appends the walkers of weight >= cand_t to
   // list A and those in [cand_t2, cand_t) to list B (0: no lists; cand_t2 = cand_t: no
-  // list B), each in 8 sub-lists of
-  // SUB_CAP entries (by workgroup index) at cand + (list * 8 + sub) * SUB_CAP;
+  // list B), each in 8 sub-lists (by workgroup index) of up to SUB_CAP entries, interleaved:
+  // entry o of sub-list sub of list `list` is at cand[o * 16 + list * 8 + sub];
   // cand_t = cut * cand_g,
   // where cut is the last step's smallest halved weight and cand_g is fed back from the
   // size of list A
@@ -88,9 +88,11 @@ constexpr int CAND_G = 32768;    // candidates the global-memory replay sorts (m
 // scratch of the global-memory replay: keys and bucketed keys (8 B each), order and bucketed
 // order (4 B each), family starts (4 B), split depths (1 B) per candidate
 constexpr size_t RSG_BYTES = (size_t)CAND_G * (8 + 8 + 4 + 4 + 4 + 1);
-constexpr int LIST_CAP = 32768;  // entries per candidate list and step
-constexpr int LIST_SUB = 8;      // sub-lists per list (spreads the appends' counters)
-constexpr int SUB_CAP = LIST_CAP / LIST_SUB;
+constexpr int LIST_SUB = 8;       // sub-lists per list (spreads the appends' counters)
+constexpr int SUB_CAP = 16384;    // entries per sub-list and step
+constexpr int LIST_CAP = LIST_SUB * SUB_CAP;
+// sub-list entries the LDS replay reads: its 16-bit positions stay below 2 * LIST_SUB * 4096
+constexpr int SUB_CAP16 = 65536 / (2 * LIST_SUB);
 constexpr int KILL_CAP = 32768;  // kills per step
 
 struct Piece {
@@ -1193,7 +1195,7 @@ __global__ __launch_bounds__(TILE) void k_cont(StepArgs<R> A) {
       c.w = w;
       c.slot = (int)i;
       c.pad = 0;
-      A.cand[(cnd ? 0 : LIST_CAP) + (tile & (LIST_SUB - 1)) * SUB_CAP + o] = c;
+      A.cand[o * (2 * LIST_SUB) + (cnd ? 0 : LIST_SUB) + (tile & (LIST_SUB - 1))] = c;
     }
   }
 }
@@ -1344,7 +1346,7 @@ PVD_DEV int replay_candidates(const StepArgs<R>& A, int K, int nlists, const int
 #pragma unroll
   for (int c = 0; c < NS; ++c) {
     const int k = c < nlists * LIST_SUB ? subn[c] : 0;
-    if (k > SUB_CAP) return RC_MANY;  // entries were lost
+    if (k > SUB_CAP16) return RC_MANY;  // entries were lost, or beyond the 16-bit positions
     pre[c + 1] = pre[c] + k;
   }
   const int nl = pre[NS];
@@ -1354,11 +1356,14 @@ PVD_DEV int replay_candidates(const StepArgs<R>& A, int K, int nlists, const int
   // position in the candidate buffer of the q-th entry (constant indices only: the
   // sub-list bounds stay in registers; an empty sub-list is passed over)
   auto phys = [&](int q) -> int {
-    int off = 0;
+    int u0 = 0, b = 0;
 #pragma unroll
     for (int u = 0; u < NS; ++u)
-      if (q >= pre[u]) off = u * SUB_CAP - pre[u];
-    return q + off;
+      if (q >= pre[u]) {
+        u0 = u;
+        b = pre[u];
+      }
+    return (q - b) * NS + u0;
   };
   const long long tb = th_base(A.thresh_lo);
   unsigned long long* key = reinterpret_cast<unsigned long long*>(buf);
@@ -1407,7 +1412,7 @@ PVD_DEV int replay_candidates(const StepArgs<R>& A, int K, int nlists, const int
         // adjacent pair of largest minimum exponent is the run's first pair
         const unsigned long long wb = (unsigned long long)__double_as_longlong(w);
         key[o] = ((MANT - (wb & MANT)) << 11) | (0x7FFull - ((wb >> 52) & 0x7FFull));
-        ord[o] = (unsigned short)pq;  // < 2 * LIST_CAP = 65536
+        ord[o] = (unsigned short)pq;  // < NS * SUB_CAP16 = 65536
       }
     }
   }
@@ -1647,13 +1652,19 @@ PVD_DEV int replay_candidates(const StepArgs<R>& A, int K, int nlists, const int
   return RC_DONE;
 }
 
+// bucket counts of the global-memory replay's counting sort (in the replay's LDS buffer), and
+// the largest bucket it ranks
+constexpr int RCG_BUCKETS = 16384;
+constexpr int RCG_BMAX = 256;
+static_assert(RCG_BUCKETS * (int)sizeof(int) <= RP_LDS && RCG_BUCKETS % RP_NT == 0, "bucket counts fit the LDS buffer");
+
 // The same closed-form replay when the kills outgrow the LDS sort (1.25 K > CAND_CAP): up to
 // CAND_G candidates, sorted and scanned through an L2-resident scratch (A.rsg) instead of LDS
 // and registers.  Keys, order, family starts and depths live in the scratch, the bucket
-// counts of the counting sort in LDS (`bcnt`, RC_BUCKETS ints).  Each octave's exclusive scan
+// counts of the counting sort in LDS (`bcnt`, RCG_BUCKETS ints).  Each octave's exclusive scan
 // of family sizes runs over chunks of RP_NT families with a carry, and every family writes the
 // kill indices of its own run.  It declines where the LDS form declines, and also when one
-// mantissa bucket holds more than 64 candidates.
+// mantissa bucket holds more than RCG_BMAX candidates.
 template <typename R>
 PVD_DEV int replay_candidates_g(const StepArgs<R>& A, int K, int nlists, const int* subn, double T0, int* bcnt,
                                 int* s_i, int* s_sh, int* hist, double& cut, bool& refined) {
@@ -1674,11 +1685,14 @@ PVD_DEV int replay_candidates_g(const StepArgs<R>& A, int K, int nlists, const i
   if (K > KILL_CAP) return RC_OTHER;
   if (!(T0 >= A.thresh_lo) || !(T0 > 0.0)) return RC_OTHER;
   auto phys = [&](int q) -> int {
-    int off = 0;
+    int u0 = 0, b = 0;
 #pragma unroll
     for (int u = 0; u < NS; ++u)
-      if (q >= pre[u]) off = u * SUB_CAP - pre[u];
-    return q + off;
+      if (q >= pre[u]) {
+        u0 = u;
+        b = pre[u];
+      }
+    return (q - b) * NS + u0;
   };
   const long long tb = th_base(A.thresh_lo);
   unsigned long long* key = reinterpret_cast<unsigned long long*>(A.rsg);
@@ -1734,33 +1748,34 @@ PVD_DEV int replay_candidates_g(const StepArgs<R>& A, int K, int nlists, const i
   const int C = s_sh[3];
   if (C > CAND_G) return RC_MANY;
   if (C == 0) return RC_FEW;
-  // 2. candidates by decreasing mantissa (then exponent): counting sort on the top 12 bits of
-  //    the 63-bit key, then each entry ranked inside its bucket
-  constexpr int BSH = 63 - 12;
-  for (int b = tid; b < RC_BUCKETS; b += NT) bcnt[b] = 0;
+  // 2. candidates by decreasing mantissa (then exponent): counting sort on the top 14 bits of
+  //    the 63-bit key, then each entry ranked inside its bucket (the entries just above the
+  //    last step's cut crowd a narrow mantissa range: 12 bits left buckets of over 64)
+  constexpr int BSH = 63 - 14;
+  constexpr int BPT = RCG_BUCKETS / NT;
+  for (int b = tid; b < RCG_BUCKETS; b += NT) bcnt[b] = 0;
   __syncthreads();
   for (int q = tid; q < C; q += NT) atomicAdd(&bcnt[(int)(key[q] >> BSH)], 1);
   __syncthreads();
   int bmax;
   {
-    int c4[4], loc = 0, mx = 0;
-#pragma unroll
-    for (int u = 0; u < 4; ++u) {
-      c4[u] = bcnt[tid * 4 + u];
-      loc += c4[u];
-      mx = max(mx, c4[u]);
+    int loc = 0, mx = 0;
+    for (int u = 0; u < BPT; ++u) {
+      const int c = bcnt[tid * BPT + u];
+      loc += c;
+      mx = max(mx, c);
     }
     int tot;
     const int ex = block_excl_scan<NT>(loc, s_i, tot);
     int st = ex;
-#pragma unroll
-    for (int u = 0; u < 4; ++u) {
-      bcnt[tid * 4 + u] = st;  // bucket starts
-      st += c4[u];
+    for (int u = 0; u < BPT; ++u) {
+      const int c = bcnt[tid * BPT + u];
+      bcnt[tid * BPT + u] = st;  // bucket starts (each thread rewrites only its own buckets)
+      st += c;
     }
     bmax = block_max<NT>(mx, s_i);
   }
-  if (bmax > 64) return RC_OTHER;
+  if (bmax > RCG_BMAX) return RC_OTHER;  // the in-bucket rank costs bucket size squared
   for (int q = tid; q < C; q += NT) {
     const int d = atomicAdd(&bcnt[(int)(key[q] >> BSH)], 1);  // bcnt ends as the bucket ends
     bkey[d] = key[q];

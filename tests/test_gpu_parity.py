@@ -164,6 +164,45 @@ def test_continuous_replay_with_more_kills_than_the_lds_sort_matches_oracle():
     np.testing.assert_allclose(wts, orc.cont_wts, rtol=1e-12, atol=0)
 
 
+def test_continuous_replay_from_a_million_walker_steady_state_matches_oracle():
+    """1M walkers, FP64: 300 Philox steps of continuous weighting (threshold 0.01) reach the
+    steady state, about 16 000 kills per step.  There the split pairs of each step pile up
+    just above its cut, so the candidates crowd a narrow mantissa range.  From those walkers and
+    weights, four steps with the reference's draws injected: the first runs the interval
+    replay (no list yet), the later ones the global-memory candidate replay; splits per step,
+    vref, weights and coordinates match the oracle's sequential loop."""
+    import pyvibdmc_amd as pv
+    from oracle import pes as P
+    from oracle.dmc import LegacyDraws, OracleDMC
+    n, steps = 1_000_000, 4
+    masses = [1837.15, 1837.15, 29156.95]
+    start = P.water_eq()[None] + np.random.RandomState(5).normal(0, 0.05, (n, 3, 3))
+    common = dict(masses=masses, num_walkers=n, delta_t=5, equil_steps=10 ** 9, wfn_every=10 ** 9,
+                  weighting='continuous', cont_wt_thresh=0.01)
+    warm = pv.DMC_Sim(atoms=['H', 'H', 'O'], potential=pv.MorseBendTriatomic(), precision='fp64', seed=3,
+                      start_structures=start, num_timesteps=300, write_files=False, chkpt_every=10 ** 9, **common)
+    warm.propagate()
+    nb_w, how_w = warm.trajectory_counts(0, 300)
+    print('warm-up splits', nb_w[-5:], 'replay', np.unique(how_w, return_counts=True), flush=True)
+    assert nb_w[-50:].min() > 10_000, nb_w[-50:]
+    c0, w0 = warm._get_walkers()
+    del warm
+    kw = dict(start_structures=c0, start_cont_wts=w0, num_timesteps=steps, **common)
+    orc = OracleDMC(potential=P.morse_bend, draws=LegacyDraws(13), **kw)
+    orc.propagate()
+    sim = pv.DMC_Sim(atoms=['H', 'H', 'O'], potential=pv.MorseBendTriatomic(), precision='fp64',
+                     rng=LegacyDraws(13), write_files=False, chkpt_every=10 ** 9, **kw)
+    sim.propagate()
+    nb, how = sim.trajectory_counts(0, steps)
+    print('splits', nb, 'replay', how, flush=True)
+    np.testing.assert_array_equal(nb, orc.rec['n_branched'])
+    assert how[0] == 2 and np.isin(how[1:], [7, 8]).all(), (nb, how)
+    np.testing.assert_allclose(sim._vref_vs_tau, orc._vref_vs_tau, rtol=0, atol=1e-10)
+    coords, wts = sim._get_walkers()
+    np.testing.assert_array_equal(coords, orc.coords)
+    np.testing.assert_allclose(wts, orc.cont_wts, rtol=1e-12, atol=0)
+
+
 def test_continuous_replay_with_a_cut_inside_the_list_matches_oracle():
     """A tenth of the walkers start in one dense cluster of near-equal heavy weights (5.0,
     spread 1e-6) and 3 % below the threshold.  After the first step (interval replay) the
