@@ -239,7 +239,7 @@ void free_sim(pvd_sim* s) {
   if (s->cap_stream) (void)hipStreamDestroy(s->cap_stream);
   void* ptrs[] = {s->coords[0], s->coords[1], s->who[0], s->who[1], s->w, s->E, s->ke, s->kmask,
                   s->ctrl, s->flags, s->gflags, s->rec, s->drec, s->accs, s->tr_vref, s->tr_pop, s->tr_dteff, s->tr_births,
-                  s->tr_deaths, s->pend, s->fin, s->klist, s->killtag, s->rstage, s->cand, s->rsg, s->asg, s->cpy, s->stage, s->parent, s->parent_w, s->desc,
+                  s->tr_deaths, s->pend, s->fin, s->klist, s->killtag, s->rstage, s->cand, s->rsg, s->ncnt, s->asg, s->cpy, s->stage, s->parent, s->parent_w, s->desc,
                   s->cnt, s->dacc, s->vsur, s->jlog, s->xc[0], s->xc[1]};
   for (void* p : ptrs)
     if (p) (void)hipFree(p);
@@ -353,6 +353,7 @@ int pvd_sim_create(const pvd_config* cfg, pvd_sim** out) {
     ALLOC(s->klist, sizeof(int) * KILL_CAP);
     ALLOC(s->cand, sizeof(Cand) * 2 * LIST_CAP);
     ALLOC(s->rsg, RSG_BYTES);
+    ALLOC(s->ncnt, CNT_BYTES);
     ALLOC(s->asg, sizeof(int) * KILL_CAP);
     ALLOC(s->cpy, sizeof(int) * 2 * KILL_CAP);
     ALLOC(s->killtag, sizeof(unsigned long long) * cap);
@@ -375,6 +376,7 @@ int pvd_sim_create(const pvd_config* cfg, pvd_sim** out) {
   if (e == hipSuccess) e = hipMemset(s->tr_births, 0, sizeof(int) * s->tmax);
   if (e == hipSuccess) e = hipMemset(s->tr_deaths, 0, sizeof(int) * s->tmax);
   if (e == hipSuccess && s->killtag) e = hipMemset(s->killtag, 0, sizeof(unsigned long long) * cap);
+  if (e == hipSuccess && s->ncnt) e = hipMemset(s->ncnt, 0, CNT_BYTES);
   if (e == hipSuccess) e = hipDeviceSynchronize();
   if (e != hipSuccess) {
     free_sim(s);

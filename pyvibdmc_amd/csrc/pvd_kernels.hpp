@@ -55,11 +55,9 @@ struct Ctrl {
   // entry o of sub-list sub of list `list` is at cand[o * 16 + list * 8 + sub];
   // cand_t = cut * cand_g,
   // where cut is the last step's smallest halved weight and cand_g is fed back from the
-  // size of list A
+  // size of list A; the entry counts of the sub-lists are in StepArgs::ncnt (see list_count)
   double cand_t;
   double cand_t2;
-  unsigned ncand[8];   // list A entries per sub-list (workgroup index mod 8)
-  unsigned ncand2[8];  // list B entries per sub-list
   int ncopy;  // split copies listed for the copy kernel (candidate replay), else 0
   float cand_g;
   int cand_b;  // steps list B stays on (it is kept after list A alone fell short)
@@ -138,6 +136,7 @@ struct StepArgs {
   double* rstage;          // staged source walkers of the split copies
   Cand* cand;              // candidate walkers of the split replay (unordered)
   unsigned char* rsg;      // scratch of the global-memory candidate replay (RSG_BYTES)
+  unsigned* ncnt;          // continuous: entry counts of the candidate sub-lists (see list_count)
   int* asg;                // per kill index: the candidate (sorted position) it copies
   int* cpy;                // per kill index: (kill slot, source slot) for the copy kernel
   PesParams pes;
@@ -170,6 +169,18 @@ constexpr int ACC_STRIDE = 32;  // unsigned words between slots
 template <typename R>
 PVD_DEV unsigned* acc_slot(const StepArgs<R>& A, int par, int b) {
   return A.accs + (size_t)(par * ACC_SLOTS + (b & (ACC_SLOTS - 1))) * ACC_STRIDE;
+}
+
+// Entry counts of the continuous candidate sub-lists (list 0 = A, 1 = B), one per 256-byte
+// line: every workgroup of the weight kernel that holds candidates adds its count to one of
+// them, and counters sharing a line serialise those adds (at 1M walkers, 3 900 workgroups
+// on one line cost the kernel 150 us).  Zero between steps: the replay reads and clears them.
+constexpr int CNT_STRIDE = 64;  // unsigned words between counters
+constexpr size_t CNT_BYTES = sizeof(unsigned) * 2 * LIST_SUB * CNT_STRIDE;
+
+template <typename R>
+PVD_DEV unsigned* list_count(const StepArgs<R>& A, int list, int sub) {
+  return A.ncnt + (size_t)(list * LIST_SUB + sub) * CNT_STRIDE;
 }
 
 // Total accepted moves of parity `par`, on every lane of the calling wave.
@@ -1181,8 +1192,8 @@ __global__ __launch_bounds__(TILE) void k_cont(StepArgs<R> A) {
     r.bad = 0;
     A.rec[tile] = r;
     const int sub = tile & (LIST_SUB - 1);
-    s_c[NW] = nc > 0 ? (int)min(atomicAdd(&C->ncand[sub], (unsigned)nc), (unsigned)SUB_CAP) : 0;
-    s_c2[NW] = nc2 > 0 ? (int)min(atomicAdd(&C->ncand2[sub], (unsigned)nc2), (unsigned)SUB_CAP) : 0;
+    s_c[NW] = nc > 0 ? (int)min(atomicAdd(list_count(A, 0, sub), (unsigned)nc), (unsigned)SUB_CAP) : 0;
+    s_c2[NW] = nc2 > 0 ? (int)min(atomicAdd(list_count(A, 1, sub), (unsigned)nc2), (unsigned)SUB_CAP) : 0;
   }
   __syncthreads();
   if (cnd || cnd2) {
@@ -2004,11 +2015,11 @@ __global__ __launch_bounds__(RP_NT) void k_cont_replay(StepArgs<R> A) {
       // the candidate count of this step; the next step's weight kernel starts from zero
       s_nc = 0;
       for (int c = 0; c < LIST_SUB; ++c) {
-        s_subn[c] = (int)min(C->ncand[c], (unsigned)SUB_CAP + 1u);
-        s_subn[LIST_SUB + c] = (int)min(C->ncand2[c], (unsigned)SUB_CAP + 1u);
+        s_subn[c] = (int)min(*list_count(A, 0, c), (unsigned)SUB_CAP + 1u);
+        s_subn[LIST_SUB + c] = (int)min(*list_count(A, 1, c), (unsigned)SUB_CAP + 1u);
         s_nc += s_subn[c];
-        C->ncand[c] = 0u;
-        C->ncand2[c] = 0u;
+        *list_count(A, 0, c) = 0u;
+        *list_count(A, 1, c) = 0u;
       }
       C->kills[par] = k;
       C->sumw[par] = a1;

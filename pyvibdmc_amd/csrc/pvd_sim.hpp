@@ -72,6 +72,7 @@ struct pvd_sim {
   double* rstage;
   pvd::Cand* cand;
   unsigned char* rsg;  // global-memory replay scratch (continuous weighting)
+  unsigned* ncnt;      // candidate sub-list counts (continuous weighting)
   int* asg;
   int* cpy;
   double* stage;
@@ -157,6 +158,7 @@ StepArgs<R> make_args(pvd_sim* s) {
   A.rstage = s->rstage;
   A.cand = s->cand;
   A.rsg = s->rsg;
+  A.ncnt = s->ncnt;
   A.asg = s->asg;
   A.cpy = s->cpy;
   std::memcpy(&A.pes, &s->cfg.pes, sizeof(PesParams));
