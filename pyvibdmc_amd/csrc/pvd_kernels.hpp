@@ -51,8 +51,8 @@ struct Ctrl {
   int ntd[2];
   // continuous split replay: the weight kernel appends the walkers of weight >= cand_t to
   // list A and those in [cand_t2, cand_t) to list B (0: no lists; cand_t2 = cand_t: no
-  // list B), each in 8 sub-lists (by workgroup index) of up to SUB_CAP entries, interleaved:
-  // entry o of sub-list sub of list `list` is at cand[o * 16 + list * 8 + sub];
+  // list B), each in 8 sub-lists (by workgroup index) of up to SUB_CAP entries: entry o of
+  // sub-list sub of list `list` is at cand[(list * 8 + sub) * SUB_CAP + o];
   // cand_t = cut * cand_g,
   // where cut is the last step's smallest halved weight and cand_g is fed back from the
   // size of list A; the entry counts of the sub-lists are in StepArgs::ncnt (see list_count)
@@ -89,8 +89,9 @@ constexpr size_t RSG_BYTES = (size_t)CAND_G * (8 + 8 + 4 + 4 + 4 + 1);
 constexpr int LIST_SUB = 8;       // sub-lists per list (spreads the appends' counters)
 constexpr int SUB_CAP = 16384;    // entries per sub-list and step
 constexpr int LIST_CAP = LIST_SUB * SUB_CAP;
-// sub-list entries the LDS replay reads: its 16-bit positions stay below 2 * LIST_SUB * 4096
-constexpr int SUB_CAP16 = 65536 / (2 * LIST_SUB);
+// sub-list entries the LDS replay reads: it keeps a candidate's position in 16 bits, the
+// sub-list (4 bits) and the offset in it (12 bits)
+constexpr int SUB_CAP16 = 4096;
 constexpr int KILL_CAP = 32768;  // kills per step
 
 struct Piece {
@@ -1206,7 +1207,7 @@ __global__ __launch_bounds__(TILE) void k_cont(StepArgs<R> A) {
       c.w = w;
       c.slot = (int)i;
       c.pad = 0;
-      A.cand[o * (2 * LIST_SUB) + (cnd ? 0 : LIST_SUB) + (tile & (LIST_SUB - 1))] = c;
+      A.cand[((cnd ? 0 : LIST_SUB) + (tile & (LIST_SUB - 1))) * SUB_CAP + o] = c;
     }
   }
 }
@@ -1364,9 +1365,9 @@ PVD_DEV int replay_candidates(const StepArgs<R>& A, int K, int nlists, const int
   if (nl <= 0) return RC_FEW;
   if (K > KILL_CAP) return RC_OTHER;
   if (!(T0 >= A.thresh_lo) || !(T0 > 0.0)) return RC_OTHER;
-  // position in the candidate buffer of the q-th entry (constant indices only: the
-  // sub-list bounds stay in registers; an empty sub-list is passed over)
-  auto phys = [&](int q) -> int {
+  // position of the q-th entry as (sub-list << 12) | offset (constant indices only: the
+  // sub-list bounds stay in registers; an empty sub-list is passed over), and the entry
+  auto pos16 = [&](int q) -> int {
     int u0 = 0, b = 0;
 #pragma unroll
     for (int u = 0; u < NS; ++u)
@@ -1374,8 +1375,9 @@ PVD_DEV int replay_candidates(const StepArgs<R>& A, int K, int nlists, const int
         u0 = u;
         b = pre[u];
       }
-    return (q - b) * NS + u0;
+    return (u0 << 12) | (q - b);
   };
+  auto cand16 = [&](int x) -> const Cand& { return A.cand[(x >> 12) * SUB_CAP + (x & 4095)]; };
   const long long tb = th_base(A.thresh_lo);
   unsigned long long* key = reinterpret_cast<unsigned long long*>(buf);
   unsigned short* ord = reinterpret_cast<unsigned short*>(buf + 8 * CAND_CAP);
@@ -1391,7 +1393,7 @@ PVD_DEV int replay_candidates(const StepArgs<R>& A, int K, int nlists, const int
   if (refined) {
     for (int b = tid; b < TH_BINS; b += NT) hist[b] = 0;
     __syncthreads();
-    for (int q = tid; q < nl; q += NT) atomicAdd(&hist[th_bin(A.cand[phys(q)].w, tb)], 1);
+    for (int q = tid; q < nl; q += NT) atomicAdd(&hist[th_bin(cand16(pos16(q)).w, tb)], 1);
     __syncthreads();
     // about 1.25 K walkers (at least 1024): they alone hold K entries
     const int lim = min(CAND_CAP, max(K + K / 4, 1024));
@@ -1404,11 +1406,11 @@ PVD_DEV int replay_candidates(const StepArgs<R>& A, int K, int nlists, const int
   __syncthreads();
   for (int q0 = 0; q0 < nl; q0 += NT) {
     const int q = q0 + tid;
-    const int pq = phys(q);
+    const int pq = pos16(q);
     double w = 0.0;
     bool in = false;
     if (q < nl) {
-      w = A.cand[pq].w;
+      w = cand16(pq).w;
       in = w >= T;
     }
     const unsigned long long bal = __ballot(in);
@@ -1423,7 +1425,7 @@ PVD_DEV int replay_candidates(const StepArgs<R>& A, int K, int nlists, const int
         // adjacent pair of largest minimum exponent is the run's first pair
         const unsigned long long wb = (unsigned long long)__double_as_longlong(w);
         key[o] = ((MANT - (wb & MANT)) << 11) | (0x7FFull - ((wb >> 52) & 0x7FFull));
-        ord[o] = (unsigned short)pq;  // < NS * SUB_CAP16 = 65536
+        ord[o] = (unsigned short)pq;
       }
     }
   }
@@ -1531,7 +1533,7 @@ PVD_DEV int replay_candidates(const StepArgs<R>& A, int K, int nlists, const int
     ej[s] = -0x40000000;
     mj[s] = 0;
     if (j < C) {
-      const Cand c = A.cand[ord[j]];
+      const Cand c = cand16(ord[j]);
       wj[s] = c.w;
       oj[s] = c.slot;
       ej[s] = dexp(c.w);
@@ -1539,7 +1541,7 @@ PVD_DEV int replay_candidates(const StepArgs<R>& A, int K, int nlists, const int
       etop = max(etop, ej[s]);
       // equal mantissas meet in every octave at or below the smaller exponent
       if (j + 1 < C && (key[j + 1] >> 11) == (key[j] >> 11))
-        tie_e = max(tie_e, min(ej[s], dexp(A.cand[ord[j + 1]].w)));
+        tie_e = max(tie_e, min(ej[s], dexp(cand16(ord[j + 1]).w)));
     }
   }
   etop = block_max<NT>(etop, s_i);
@@ -1604,7 +1606,7 @@ PVD_DEV int replay_candidates(const StepArgs<R>& A, int K, int nlists, const int
   if (tie_e >= Es) return RC_OTHER;
   const int p = s_sh[0], startp = s_sh[1];
   const int r = K - startp;
-  const Cand cp = A.cand[ord[p]];
+  const Cand cp = cand16(ord[p]);
   const int Dp = dexp(cp.w) - Es;
   const int m = 1 << Dp;
   if (Dp > 13) return RC_OTHER;  // the family's slots must fit the scratch
@@ -1646,7 +1648,7 @@ PVD_DEV int replay_candidates(const StepArgs<R>& A, int K, int nlists, const int
       o = cp.slot;
       wn = ldexp(cp.w, -(Dp + 1));
     } else {
-      const Cand c = A.cand[ord[j]];
+      const Cand c = cand16(ord[j]);
       o = c.slot;
       wn = ldexp(c.w, -(int)dep[j]);
     }
@@ -1671,17 +1673,28 @@ static_assert(RCG_BUCKETS * (int)sizeof(int) <= RP_LDS && RCG_BUCKETS % RP_NT ==
 
 // The same closed-form replay when the kills outgrow the LDS sort (1.25 K > CAND_CAP): up to
 // CAND_G candidates, sorted and scanned through an L2-resident scratch (A.rsg) instead of LDS
-// and registers.  Keys, order, family starts and depths live in the scratch, the bucket
-// counts of the counting sort in LDS (`bcnt`, RCG_BUCKETS ints).  Each octave's exclusive scan
-// of family sizes runs over chunks of RP_NT families with a carry, and every family writes the
-// kill indices of its own run.  It declines where the LDS form declines, and also when one
-// mantissa bucket holds more than RCG_BMAX candidates.
+// and registers.  One workgroup runs it, so each pass over the scratch is bound by the latency
+// of its loads: every streaming loop issues RG_U independent loads per thread before it uses
+// them, and the octave scans give each thread a contiguous run of candidates (one block scan
+// per octave).  The sort key holds the weight's exponent and mantissa whole (the weight is
+// rebuilt from it) and the candidate's walker slot travels with it, so no pass after the
+// collection reads the lists again.  Bucket counts of the counting sort live in LDS (`bcnt`,
+// RCG_BUCKETS ints).  It declines where the LDS form declines, and also when one mantissa
+// bucket holds more than RCG_BMAX candidates.
+constexpr int RG_U = 4;  // loads in flight per thread in the streaming loops
+
+PVD_DEV double key_weight(unsigned long long k) {
+  constexpr unsigned long long MANT = (1ull << 52) - 1;
+  return __longlong_as_double((long long)(((0x7FFull - (k & 0x7FFull)) << 52) | (MANT - (k >> 11))));
+}
+
 template <typename R>
 PVD_DEV int replay_candidates_g(const StepArgs<R>& A, int K, int nlists, const int* subn, double T0, int* bcnt,
                                 int* s_i, int* s_sh, int* hist, double& cut, bool& refined) {
   constexpr int NT = RP_NT;
   constexpr unsigned long long MANT = (1ull << 52) - 1;
   constexpr int NS = 2 * LIST_SUB;
+  constexpr int U = RG_U;
   const int tid = threadIdx.x;
   int pre[NS + 1];
   pre[0] = 0;
@@ -1703,16 +1716,17 @@ PVD_DEV int replay_candidates_g(const StepArgs<R>& A, int K, int nlists, const i
         u0 = u;
         b = pre[u];
       }
-    return (q - b) * NS + u0;
+    return u0 * SUB_CAP + (q - b);
   };
   const long long tb = th_base(A.thresh_lo);
+  // scratch: keys, bucketed keys (8 B each), slots, bucketed slots, family-p slots (4 B
+  // each), split depths (1 B); key = (inverted mantissa << 11) | inverted exponent
   unsigned long long* key = reinterpret_cast<unsigned long long*>(A.rsg);
   unsigned long long* bkey = key + CAND_G;
   int* ord = reinterpret_cast<int*>(bkey + CAND_G);
   int* bord = ord + CAND_G;
   int* start = bord + CAND_G;
   unsigned char* dep = reinterpret_cast<unsigned char*>(start + CAND_G);
-  // key -> exponent and mantissa of the candidate's weight (the key is built below)
   auto kexp = [](unsigned long long k) { return (int)(0x7FFull - (k & 0x7FFull)) - 1023; };
   auto kmant = [](unsigned long long k) { return MANT - (k >> 11); };
   // 1. the cut: T0 when the list fits the sort, else the lowest histogram edge with at most
@@ -1729,44 +1743,74 @@ PVD_DEV int replay_candidates_g(const StepArgs<R>& A, int K, int nlists, const i
     if (over < 0 || over >= TH_BINS - 1) return RC_MANY;
     T = fmax(T0, th_edge(A.thresh_lo, over + 1));
   }
-  // the walkers >= T, in any order (wave ballots onto one counter)
+  // the walkers >= T, in any order (wave ballots onto one counter): key and walker slot, and
+  // the range of the keys
   if (tid == 0) s_sh[3] = 0;
   __syncthreads();
-  for (int q0 = 0; q0 < nl; q0 += NT) {
-    const int q = q0 + tid;
-    const int pq = q < nl ? phys(q) : 0;
-    double w = 0.0;
-    bool in = false;
-    if (q < nl) {
-      w = A.cand[pq].w;
-      in = w >= T;
+  unsigned long long kmin = ~0ull, kmax = 0ull;
+  for (int q0 = 0; q0 < nl; q0 += U * NT) {
+    Cand c[U];
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+      const int q = q0 + u * NT + tid;
+      if (q < nl) {
+        c[u] = A.cand[phys(q)];
+      } else {
+        c[u].w = -1.0;
+        c[u].slot = 0;
+      }
     }
-    const unsigned long long bal = __ballot(in);
-    if (bal) {
-      const int lead = __ffsll((long long)bal) - 1;
-      int base = 0;
-      if (lane_id() == lead) base = atomicAdd(&s_sh[3], __popcll(bal));
-      base = __shfl(base, lead, PVD_WAVE);
-      const int o = base + __popcll(bal & ((1ull << lane_id()) - 1ull));
-      if (in && o < CAND_G) {
-        const unsigned long long wb = (unsigned long long)__double_as_longlong(w);
-        key[o] = ((MANT - (wb & MANT)) << 11) | (0x7FFull - ((wb >> 52) & 0x7FFull));
-        ord[o] = pq;
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+      const bool in = c[u].w >= T;
+      const unsigned long long bal = __ballot(in);
+      if (bal) {
+        const int lead = __ffsll((long long)bal) - 1;
+        int base = 0;
+        if (lane_id() == lead) base = atomicAdd(&s_sh[3], __popcll(bal));
+        base = __shfl(base, lead, PVD_WAVE);
+        const int o = base + __popcll(bal & ((1ull << lane_id()) - 1ull));
+        if (in && o < CAND_G) {
+          const unsigned long long wb = (unsigned long long)__double_as_longlong(c[u].w);
+          const unsigned long long k = ((MANT - (wb & MANT)) << 11) | (0x7FFull - ((wb >> 52) & 0x7FFull));
+          key[o] = k;
+          ord[o] = c[u].slot;
+          kmin = k < kmin ? k : kmin;
+          kmax = k > kmax ? k : kmax;
+        }
       }
     }
   }
-  __syncthreads();
+  {
+    unsigned long long* s_k = reinterpret_cast<unsigned long long*>(hist);  // free after step 1
+    kmax = block_max<NT>(kmax, s_k);
+    kmin = ~block_max<NT>(~kmin, s_k);
+  }
   const int C = s_sh[3];
   if (C > CAND_G) return RC_MANY;
   if (C == 0) return RC_FEW;
-  // 2. candidates by decreasing mantissa (then exponent): counting sort on the top 14 bits of
-  //    the 63-bit key, then each entry ranked inside its bucket (the entries just above the
-  //    last step's cut crowd a narrow mantissa range: 12 bits left buckets of over 64)
-  constexpr int BSH = 63 - 14;
+  // 2. candidates by decreasing mantissa (then exponent): counting sort on 14 bits of the key
+  //    above its minimum (an order-preserving map of the keys' own range onto the buckets: the
+  //    candidates of a steady state lie in a band about a tenth of an octave wide, which the
+  //    top mantissa bits would crowd into a tenth of the buckets), then each entry ranked
+  //    inside its bucket
   constexpr int BPT = RCG_BUCKETS / NT;
+  const unsigned long long kspan = kmax - kmin;
+  const int bsh = (kspan >> 14) == 0ull ? 0 : 64 - __clzll((long long)kspan) - 14;
+  auto bucket = [&](unsigned long long k) { return (int)((k - kmin) >> bsh); };  // < RCG_BUCKETS
   for (int b = tid; b < RCG_BUCKETS; b += NT) bcnt[b] = 0;
   __syncthreads();
-  for (int q = tid; q < C; q += NT) atomicAdd(&bcnt[(int)(key[q] >> BSH)], 1);
+  for (int q0 = 0; q0 < C; q0 += U * NT) {
+    unsigned long long kk[U];
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+      const int q = q0 + u * NT + tid;
+      kk[u] = q < C ? key[q] : 0ull;
+    }
+#pragma unroll
+    for (int u = 0; u < U; ++u)
+      if (q0 + u * NT + tid < C) atomicAdd(&bcnt[bucket(kk[u])], 1);
+  }
   __syncthreads();
   int bmax;
   {
@@ -1787,31 +1831,85 @@ PVD_DEV int replay_candidates_g(const StepArgs<R>& A, int K, int nlists, const i
     bmax = block_max<NT>(mx, s_i);
   }
   if (bmax > RCG_BMAX) return RC_OTHER;  // the in-bucket rank costs bucket size squared
-  for (int q = tid; q < C; q += NT) {
-    const int d = atomicAdd(&bcnt[(int)(key[q] >> BSH)], 1);  // bcnt ends as the bucket ends
-    bkey[d] = key[q];
-    bord[d] = ord[q];
+  for (int q0 = 0; q0 < C; q0 += U * NT) {
+    unsigned long long kk[U];
+    int oo[U];
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+      const int q = q0 + u * NT + tid;
+      kk[u] = q < C ? key[q] : 0ull;
+      oo[u] = q < C ? ord[q] : 0;
+    }
+#pragma unroll
+    for (int u = 0; u < U; ++u)
+      if (q0 + u * NT + tid < C) {
+        const int d = atomicAdd(&bcnt[bucket(kk[u])], 1);  // bcnt ends as the bucket ends
+        bkey[d] = kk[u];
+        bord[d] = oo[u];
+      }
   }
   __syncthreads();
-  for (int q = tid; q < C; q += NT) {
-    const unsigned long long kq = bkey[q];
-    const int bk = (int)(kq >> BSH);
-    const int bs = bk > 0 ? bcnt[bk - 1] : 0, be = bcnt[bk];
-    int rank = 0;
-    for (int i = bs; i < be; ++i) {
-      const unsigned long long ki = bkey[i];
-      rank += (ki < kq || (ki == kq && i < q)) ? 1 : 0;
+  // ranks: U entries per thread at a time, the first RK keys of each one's bucket in flight
+  constexpr int RK = 4;
+  for (int q0 = 0; q0 < C; q0 += U * NT) {
+    unsigned long long kq[U];
+    int oq[U], bs[U], be[U], rank[U];
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+      const int q = q0 + u * NT + tid;
+      kq[u] = q < C ? bkey[q] : 0ull;
+      oq[u] = q < C ? bord[q] : 0;
     }
-    key[bs + rank] = kq;
-    ord[bs + rank] = bord[q];
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+      const int q = q0 + u * NT + tid;
+      const int bk = q < C ? bucket(kq[u]) : 0;
+      bs[u] = q < C ? (bk > 0 ? bcnt[bk - 1] : 0) : 0;
+      be[u] = q < C ? bcnt[bk] : 0;
+      rank[u] = 0;
+    }
+#pragma unroll
+    for (int i = 0; i < RK; ++i)
+#pragma unroll
+      for (int u = 0; u < U; ++u) {
+        const int x = bs[u] + i, q = q0 + u * NT + tid;
+        if (x < be[u]) {
+          const unsigned long long ki = bkey[x];
+          rank[u] += (ki < kq[u] || (ki == kq[u] && x < q)) ? 1 : 0;
+        }
+      }
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+      const int q = q0 + u * NT + tid;
+      for (int x = bs[u] + RK; x < be[u]; ++x) {
+        const unsigned long long ki = bkey[x];
+        rank[u] += (ki < kq[u] || (ki == kq[u] && x < q)) ? 1 : 0;
+      }
+      if (q < C) {
+        key[bs[u] + rank[u]] = kq[u];
+        ord[bs[u] + rank[u]] = oq[u];
+      }
+    }
   }
   __syncthreads();
   // 3. the top octave, and the highest octave in which two equal mantissas meet
   int etop = -0x40000000, tie_e = -0x40000000;
-  for (int j = tid; j < C; j += NT) {
-    const unsigned long long kj = key[j];
-    etop = max(etop, kexp(kj));
-    if (j + 1 < C && (key[j + 1] >> 11) == (kj >> 11)) tie_e = max(tie_e, min(kexp(kj), kexp(key[j + 1])));
+  for (int j0 = 0; j0 < C; j0 += U * NT) {
+    unsigned long long kk[U], kn[U];
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+      const int j = j0 + u * NT + tid;
+      kk[u] = j < C ? key[j] : 0ull;
+      kn[u] = j + 1 < C ? key[j + 1] : 0ull;
+    }
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+      const int j = j0 + u * NT + tid;
+      if (j < C) {
+        etop = max(etop, kexp(kk[u]));
+        if (j + 1 < C && (kn[u] >> 11) == (kk[u] >> 11)) tie_e = max(tie_e, min(kexp(kk[u]), kexp(kn[u])));
+      }
+    }
   }
   etop = block_max<NT>(etop, s_i);
   tie_e = block_max<NT>(tie_e, s_i);
@@ -1819,100 +1917,157 @@ PVD_DEV int replay_candidates_g(const StepArgs<R>& A, int K, int nlists, const i
   const unsigned long long mT = (unsigned long long)__double_as_longlong(T) & MANT;
   if (etop - eT > 40) return RC_OTHER;
   // 4. octaves from the top: each family's run of kill indices, until K are used; in the
-  //    octave of T only the entries of value >= T count
+  //    octave of T only the entries of value >= T count.  Thread tid holds the candidates
+  //    [j0, j1) in sorted order: its entry count, one block scan, then its families' runs.
+  auto fam = [&](unsigned long long kj, int E) -> int {
+    const int ej = kexp(kj);
+    if (!(ej >= E && (E > eT || kmant(kj) >= mT))) return 0;
+    const int d = ej - E;
+    return d >= 20 ? KILL_CAP + 1 : min(1 << d, KILL_CAP + 1);
+  };
+  const int P = (C + NT - 1) / NT;
+  const int j0 = min(C, tid * P), j1 = min(C, j0 + P);
   int base = 0, Es = 0;
   bool found = false;
   for (int E = etop; E >= eT; --E) {
-    int carry = base;
-    for (int q0 = 0; q0 < C; q0 += NT) {
-      const int j = q0 + tid;
-      int v = 0;
-      if (j < C) {
-        const unsigned long long kj = key[j];
-        const int ej = kexp(kj);
-        if (ej >= E && (E > eT || kmant(kj) >= mT)) {
-          const int d = ej - E;
-          v = d >= 20 ? KILL_CAP + 1 : min(1 << d, KILL_CAP + 1);
-        }
-      }
-      int tot;
-      const int ex = block_excl_scan<NT>(v, s_i, tot);
-      const int st = carry + ex;
-      if (v > 0 && st < K) {
-        const int hi = min(st + v, K);
-        for (int t = st; t < hi; ++t) A.asg[t] = j;
-        if (st + v >= K) {
-          s_sh[0] = j;
-          s_sh[1] = st;
-        }
-      }
-      carry += tot;
+    int loc = 0;
+    for (int jb = j0; jb < j1; jb += U) {
+      unsigned long long kk[U];
+#pragma unroll
+      for (int u = 0; u < U; ++u) kk[u] = jb + u < j1 ? key[jb + u] : 0ull;
+#pragma unroll
+      for (int u = 0; u < U; ++u)
+        if (jb + u < j1) loc += fam(kk[u], E);
     }
+    int tot;
+    const int ex = block_excl_scan<NT>(loc, s_i, tot);
+    int st = base + ex;
+    for (int jb = j0; jb < j1 && st < K; jb += U) {
+      unsigned long long kk[U];
+#pragma unroll
+      for (int u = 0; u < U; ++u) kk[u] = jb + u < j1 ? key[jb + u] : 0ull;
+#pragma unroll
+      for (int u = 0; u < U; ++u) {
+        const int j = jb + u;
+        const int v = j < j1 ? fam(kk[u], E) : 0;
+        if (v > 0 && st < K) {
+          const int hi = min(st + v, K);
+          for (int t = st; t < hi; ++t) A.asg[t] = j;
+          if (st + v >= K) {
+            s_sh[0] = j;
+            s_sh[1] = st;
+          }
+        }
+        st += v;
+      }
+    }
+    base += tot;
     __syncthreads();
-    if (carry >= K) {
+    if (base >= K) {
       Es = E;
       found = true;
       break;
     }
-    base = carry;
   }
   if (!found) return RC_FEW;
   if (tie_e >= Es) return RC_OTHER;
   const int p = s_sh[0], startp = s_sh[1];
   const int r = K - startp;
-  const Cand cp = A.cand[ord[p]];
-  const int Dp = dexp(cp.w) - Es;
+  const double cpw = key_weight(key[p]);
+  const int cps = ord[p];
+  const int Dp = dexp(cpw) - Es;
   const int m = 1 << Dp;
   if (Dp > 13) return RC_OTHER;  // the family's slots must fit the scratch
-  for (int j = tid; j < C; j += NT) {
-    const int ej = kexp(key[j]);
-    dep[j] = (unsigned char)(ej < Es ? 0 : ej - Es + (j < p ? 1 : 0));
+  // split depth of each candidate (its own thread reads it again in step 6's last loop;
+  //    step 5's barriers order it before the reads of other threads)
+  for (int jb = j0; jb < j1; jb += U) {
+    unsigned long long kk[U];
+#pragma unroll
+    for (int u = 0; u < U; ++u) kk[u] = jb + u < j1 ? key[jb + u] : 0ull;
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+      const int j = jb + u;
+      if (j < j1) {
+        const int ej = kexp(kk[u]);
+        dep[j] = (unsigned char)(ej < Es ? 0 : ej - Es + (j < p ? 1 : 0));
+      }
+    }
   }
-  // 5. the slots of family p: the kill slots of its full levels, then its own
+  // 5. the slots of family p: the kill slots of its full levels (any order: the weights
+  //    below rank them by slot), then its own
   int* sp = start;
-  int cnt = 0;
-  for (int t0 = 0; t0 < startp; t0 += NT) {
-    const int t = t0 + tid;
-    const bool mem = t < startp && A.asg[t] == p;
-    int tot;
-    const int ex = block_excl_scan<NT>(mem ? 1 : 0, s_i, tot);
-    if (mem && cnt + ex < m) sp[cnt + ex] = A.klist[t];
-    cnt += tot;
+  if (tid == 0) s_sh[3] = 0;
+  __syncthreads();
+  for (int t0 = 0; t0 < startp; t0 += U * NT) {
+    int aa[U];
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+      const int t = t0 + u * NT + tid;
+      aa[u] = t < startp ? A.asg[t] : -1;
+    }
+#pragma unroll
+    for (int u = 0; u < U; ++u)
+      if (aa[u] == p) {
+        const int o = atomicAdd(&s_sh[3], 1);
+        if (o < m - 1) sp[o] = A.klist[t0 + u * NT + tid];
+      }
   }
-  if (cnt + 1 != m) return RC_OTHER;  // not reached: family p has 2^Dp entries
-  if (tid == 0) sp[m - 1] = cp.slot;
+  __syncthreads();
+  if (s_sh[3] + 1 != m) return RC_OTHER;  // not reached: family p has 2^Dp entries
+  if (tid == 0) sp[m - 1] = cps;
   __syncthreads();
   // 6. weights in place, and the list of copies (kill slot, source) for the copy kernel
   for (int i = tid; i < m; i += NT) {
     const int sl = sp[i];
     int rank = 0;
     for (int k = 0; k < m; ++k) rank += sp[k] < sl ? 1 : 0;
-    A.w[sl] = ldexp(cp.w, -(Dp + (rank < r ? 1 : 0)));
+    A.w[sl] = ldexp(cpw, -(Dp + (rank < r ? 1 : 0)));
   }
-  for (int t = tid; t < K; t += NT) {
-    const int j = A.asg[t];
-    const int ks = A.klist[t];
-    int o;
-    double wn;
-    if (j == p) {
-      o = cp.slot;
-      wn = ldexp(cp.w, -(Dp + 1));
-    } else {
-      const Cand c = A.cand[ord[j]];
-      o = c.slot;
-      wn = ldexp(c.w, -(int)dep[j]);
+  for (int t0 = 0; t0 < K; t0 += U * NT) {
+    int jj[U], ks[U];
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+      const int t = t0 + u * NT + tid;
+      jj[u] = t < K ? A.asg[t] : p;
+      ks[u] = t < K ? A.klist[t] : 0;
     }
-    if (j != p || t >= startp) A.w[ks] = wn;
-    A.cpy[2 * t] = ks;
-    A.cpy[2 * t + 1] = o;
-  }
-  for (int j = tid; j < C; j += NT) {
-    if (j != p && dep[j] > 0) {
-      const Cand c = A.cand[ord[j]];
-      A.w[c.slot] = ldexp(c.w, -(int)dep[j]);
+    unsigned long long kk[U];
+    int oo[U], dd[U];
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+      const bool own = jj[u] == p;
+      kk[u] = own ? 0ull : key[jj[u]];
+      oo[u] = own ? cps : ord[jj[u]];
+      dd[u] = own ? 0 : dep[jj[u]];
+    }
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+      const int t = t0 + u * NT + tid;
+      if (t < K) {
+        const bool own = jj[u] == p;
+        const double wn = own ? ldexp(cpw, -(Dp + 1)) : ldexp(key_weight(kk[u]), -dd[u]);
+        if (!own || t >= startp) A.w[ks[u]] = wn;
+        A.cpy[2 * t] = ks[u];
+        A.cpy[2 * t + 1] = oo[u];
+      }
     }
   }
-  cut = ldexp(cp.w, -Dp);
+  for (int jb = j0; jb < j1; jb += U) {
+    unsigned long long kk[U];
+    int oo[U], dd[U];
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+      const int j = jb + u;
+      const bool in = j < j1;
+      kk[u] = in ? key[j] : 0ull;
+      oo[u] = in ? ord[j] : 0;
+      dd[u] = in ? (int)dep[j] : 0;
+    }
+#pragma unroll
+    for (int u = 0; u < U; ++u)
+      if (jb + u < j1 && jb + u != p && dd[u] > 0) A.w[oo[u]] = ldexp(key_weight(kk[u]), -dd[u]);
+  }
+  cut = ldexp(cpw, -Dp);
   return RC_DONE;
 }
 
