@@ -61,6 +61,17 @@ struct Ctrl {
   int ncopy;  // split copies listed for the copy kernel (candidate replay), else 0
   float cand_g;
   int cand_b;  // steps list B stays on (it is kept after list A alone fell short)
+  // the global-memory candidate replay leaves its kill slots' sources and weights to the copy
+  // kernel: gc_n sorted candidates in the replay scratch (0: the copies are (slot, source)
+  // pairs in cpy), and the family where the kills ran out (sorted index, slot, weight, split
+  // depth, first kill index in its last octave)
+  int gc_n;
+  int gc_p;
+  int gc_slot;
+  int gc_dp;
+  int gc_start;
+  int pad2;
+  double gc_w;
 };
 
 struct TileRec {
@@ -1683,6 +1694,27 @@ static_assert(RCG_BUCKETS * (int)sizeof(int) <= RP_LDS && RCG_BUCKETS % RP_NT ==
 // bucket holds more than RCG_BMAX candidates.
 constexpr int RG_U = 4;  // loads in flight per thread in the streaming loops
 
+// the replay scratch: keys, bucketed keys (8 B each), walker slots, bucketed slots, family-p
+// slots (4 B each), split depths (1 B), CAND_G of each
+struct RsgView {
+  unsigned long long* key;
+  unsigned long long* bkey;
+  int* ord;
+  int* bord;
+  int* start;
+  unsigned char* dep;
+};
+PVD_DEV RsgView rsg_view(unsigned char* rsg) {
+  RsgView v;
+  v.key = reinterpret_cast<unsigned long long*>(rsg);
+  v.bkey = v.key + CAND_G;
+  v.ord = reinterpret_cast<int*>(v.bkey + CAND_G);
+  v.bord = v.ord + CAND_G;
+  v.start = v.bord + CAND_G;
+  v.dep = reinterpret_cast<unsigned char*>(v.start + CAND_G);
+  return v;
+}
+
 PVD_DEV double key_weight(unsigned long long k) {
   constexpr unsigned long long MANT = (1ull << 52) - 1;
   return __longlong_as_double((long long)(((0x7FFull - (k & 0x7FFull)) << 52) | (MANT - (k >> 11))));
@@ -1719,14 +1751,14 @@ PVD_DEV int replay_candidates_g(const StepArgs<R>& A, int K, int nlists, const i
     return u0 * SUB_CAP + (q - b);
   };
   const long long tb = th_base(A.thresh_lo);
-  // scratch: keys, bucketed keys (8 B each), slots, bucketed slots, family-p slots (4 B
-  // each), split depths (1 B); key = (inverted mantissa << 11) | inverted exponent
-  unsigned long long* key = reinterpret_cast<unsigned long long*>(A.rsg);
-  unsigned long long* bkey = key + CAND_G;
-  int* ord = reinterpret_cast<int*>(bkey + CAND_G);
-  int* bord = ord + CAND_G;
-  int* start = bord + CAND_G;
-  unsigned char* dep = reinterpret_cast<unsigned char*>(start + CAND_G);
+  // key = (inverted mantissa << 11) | inverted exponent; ord holds the walker slot
+  const RsgView rv = rsg_view(A.rsg);
+  unsigned long long* key = rv.key;
+  unsigned long long* bkey = rv.bkey;
+  int* ord = rv.ord;
+  int* bord = rv.bord;
+  int* start = rv.start;
+  unsigned char* dep = rv.dep;
   auto kexp = [](unsigned long long k) { return (int)(0x7FFull - (k & 0x7FFull)) - 1023; };
   auto kmant = [](unsigned long long k) { return MANT - (k >> 11); };
   // 1. the cut: T0 when the list fits the sort, else the lowest histogram edge with at most
@@ -1978,8 +2010,7 @@ PVD_DEV int replay_candidates_g(const StepArgs<R>& A, int K, int nlists, const i
   const int Dp = dexp(cpw) - Es;
   const int m = 1 << Dp;
   if (Dp > 13) return RC_OTHER;  // the family's slots must fit the scratch
-  // split depth of each candidate (its own thread reads it again in step 6's last loop;
-  //    step 5's barriers order it before the reads of other threads)
+  // split depth of each candidate (read by the copy kernel)
   for (int jb = j0; jb < j1; jb += U) {
     unsigned long long kk[U];
 #pragma unroll
@@ -2016,56 +2047,22 @@ PVD_DEV int replay_candidates_g(const StepArgs<R>& A, int K, int nlists, const i
   if (s_sh[3] + 1 != m) return RC_OTHER;  // not reached: family p has 2^Dp entries
   if (tid == 0) sp[m - 1] = cps;
   __syncthreads();
-  // 6. weights in place, and the list of copies (kill slot, source) for the copy kernel
+  // 6. the weights of family p's own slots; every other write (the kill slots' sources and
+  //    weights, the split candidates' own weights) is the copy kernel's, chip-wide
   for (int i = tid; i < m; i += NT) {
     const int sl = sp[i];
     int rank = 0;
     for (int k = 0; k < m; ++k) rank += sp[k] < sl ? 1 : 0;
     A.w[sl] = ldexp(cpw, -(Dp + (rank < r ? 1 : 0)));
   }
-  for (int t0 = 0; t0 < K; t0 += U * NT) {
-    int jj[U], ks[U];
-#pragma unroll
-    for (int u = 0; u < U; ++u) {
-      const int t = t0 + u * NT + tid;
-      jj[u] = t < K ? A.asg[t] : p;
-      ks[u] = t < K ? A.klist[t] : 0;
-    }
-    unsigned long long kk[U];
-    int oo[U], dd[U];
-#pragma unroll
-    for (int u = 0; u < U; ++u) {
-      const bool own = jj[u] == p;
-      kk[u] = own ? 0ull : key[jj[u]];
-      oo[u] = own ? cps : ord[jj[u]];
-      dd[u] = own ? 0 : dep[jj[u]];
-    }
-#pragma unroll
-    for (int u = 0; u < U; ++u) {
-      const int t = t0 + u * NT + tid;
-      if (t < K) {
-        const bool own = jj[u] == p;
-        const double wn = own ? ldexp(cpw, -(Dp + 1)) : ldexp(key_weight(kk[u]), -dd[u]);
-        if (!own || t >= startp) A.w[ks[u]] = wn;
-        A.cpy[2 * t] = ks[u];
-        A.cpy[2 * t + 1] = oo[u];
-      }
-    }
-  }
-  for (int jb = j0; jb < j1; jb += U) {
-    unsigned long long kk[U];
-    int oo[U], dd[U];
-#pragma unroll
-    for (int u = 0; u < U; ++u) {
-      const int j = jb + u;
-      const bool in = j < j1;
-      kk[u] = in ? key[j] : 0ull;
-      oo[u] = in ? ord[j] : 0;
-      dd[u] = in ? (int)dep[j] : 0;
-    }
-#pragma unroll
-    for (int u = 0; u < U; ++u)
-      if (jb + u < j1 && jb + u != p && dd[u] > 0) A.w[oo[u]] = ldexp(key_weight(kk[u]), -dd[u]);
+  if (tid == 0) {
+    Ctrl* Cw = A.ctrl;
+    Cw->gc_n = C;
+    Cw->gc_p = p;
+    Cw->gc_slot = cps;
+    Cw->gc_dp = Dp;
+    Cw->gc_start = startp;
+    Cw->gc_w = cpw;
   }
   cut = ldexp(cpw, -Dp);
   return RC_DONE;
@@ -2167,6 +2164,7 @@ __global__ __launch_bounds__(RP_NT) void k_cont_replay(StepArgs<R> A) {
       s_a0n = a0n;
       s_a1n = a1n;
       C->ncopy = 0;
+      C->gc_n = 0;
       // the candidate count of this step; the next step's weight kernel starts from zero
       s_nc = 0;
       for (int c = 0; c < LIST_SUB; ++c) {
@@ -2580,14 +2578,41 @@ __global__ __launch_bounds__(RP_NT) void k_cont_replay(StepArgs<R> A) {
 
 // The split copies the candidate replay lists: kill slot <- source walker (coordinates,
 // energy, descendant index).  Sources are never kill slots, so the copies are independent
-// and spread over the whole chip.
+// and spread over the whole chip.  After the global-memory replay (gc_n > 0) the copy of
+// kill index t takes its family from asg[t] and its slot from klist[t], and writes the kill
+// slot's weight; thread t also writes sorted candidate t's own weight when it was split.
+// Kill slots, split candidates and family p's slots are distinct walkers, so every weight
+// written here has one writer.
 template <typename R>
 __global__ __launch_bounds__(TILE) void k_cont_copy(StepArgs<R> A) {
   constexpr int B = 9;  // components per batch: all loads of a batch before its stores
   const Ctrl* C = A.ctrl;
-  const int t = blockIdx.x * TILE + threadIdx.x;  // < KILL_CAP: the list load is in bounds
-  const int ks = A.cpy[2 * t], o = A.cpy[2 * t + 1];
-  if (C->status != ST_OK || t >= C->ncopy) return;
+  const int t = blockIdx.x * TILE + threadIdx.x;  // < KILL_CAP: the list loads are in bounds
+  if (C->status != ST_OK) return;
+  const int gn = C->gc_n;
+  int ks, o;
+  if (gn > 0) {
+    const RsgView rv = rsg_view(A.rsg);
+    const int gp = C->gc_p;
+    if (t < gn && t != gp) {
+      const int d = rv.dep[t];
+      if (d > 0) A.w[rv.ord[t]] = ldexp(key_weight(rv.key[t]), -d);
+    }
+    if (t >= C->ncopy) return;
+    ks = A.klist[t];
+    const int j = A.asg[t];
+    if (j == gp) {
+      o = C->gc_slot;
+      if (t >= C->gc_start) A.w[ks] = ldexp(C->gc_w, -(C->gc_dp + 1));
+    } else {
+      o = rv.ord[j];
+      A.w[ks] = ldexp(key_weight(rv.key[j]), -(int)rv.dep[j]);
+    }
+  } else {
+    if (t >= C->ncopy) return;
+    ks = A.cpy[2 * t];
+    o = A.cpy[2 * t + 1];
+  }
   const double e = A.E[o];
   const int who = A.who_out ? A.who_out[o] : 0;
   for (int p0 = 0; p0 < A.np; p0 += B) {
