@@ -1761,19 +1761,34 @@ PVD_DEV int replay_candidates_g(const StepArgs<R>& A, int K, int nlists, const i
   unsigned char* dep = rv.dep;
   auto kexp = [](unsigned long long k) { return (int)(0x7FFull - (k & 0x7FFull)) - 1023; };
   auto kmant = [](unsigned long long k) { return MANT - (k >> 11); };
-  // 1. the cut: T0 when the list fits the sort, else the lowest histogram edge with at most
-  //    about 1.25 K entries at or above it
+  // 1. the cut: T0, or, when the lists hold over a quarter more than about 1.25 K walkers
+  //    (at a steady state list A holds the last step's 2 K split pieces), the lowest
+  //    histogram edge with at most 1.25 K entries at or above it: every later pass is over
+  //    the walkers >= the cut only
   double T = T0;
-  refined = nl > CAND_G;
+  const int lim = min(CAND_G, max(K + K / 4, 1024));
+  refined = nl > lim + lim / 4;
   if (refined) {
     for (int b = tid; b < TH_BINS; b += NT) hist[b] = 0;
     __syncthreads();
-    for (int q = tid; q < nl; q += NT) atomicAdd(&hist[th_bin(A.cand[phys(q)].w, tb)], 1);
+    for (int q0 = 0; q0 < nl; q0 += U * NT) {
+      double wq[U];
+#pragma unroll
+      for (int u = 0; u < U; ++u) {
+        const int q = q0 + u * NT + tid;
+        wq[u] = q < nl ? A.cand[phys(q)].w : 0.0;
+      }
+#pragma unroll
+      for (int u = 0; u < U; ++u)
+        if (q0 + u * NT + tid < nl) atomicAdd(&hist[th_bin(wq[u], tb)], 1);
+    }
     __syncthreads();
-    const int lim = min(CAND_G, max(K + K / 4, 1024));
     const int over = th_cross(hist, lim, true, s_i, s_sh);
     if (over < 0 || over >= TH_BINS - 1) return RC_MANY;
-    T = fmax(T0, th_edge(A.thresh_lo, over + 1));
+    // and no higher than the edge that keeps K walkers: one bin (0.4 % of an octave) can
+    // hold thousands when the weights are bunched, as in the first steps of a run
+    const int reach = th_cross(hist, K, false, s_i, s_sh);
+    if (reach >= 0) T = fmax(T0, th_edge(A.thresh_lo, min(over + 1, reach)));
   }
   // the walkers >= T, in any order (wave ballots onto one counter): key and walker slot, and
   // the range of the keys
@@ -2244,18 +2259,28 @@ __global__ __launch_bounds__(RP_NT) void k_cont_replay(StepArgs<R> A) {
   {
     double cut = 0.0;
     bool refined = false;
-    // more kills than the LDS sort serves: the global-memory form
-    const bool big = K + K / 4 > CAND_CAP && A.rsg != nullptr;
+    // more kills than the LDS sort serves: the global-memory form; it also takes the lists
+    // the LDS form declines as too long
+    bool big = K + K / 4 > CAND_CAP && A.rsg != nullptr;
     auto replay = [&](int nlists, double T0) {
       return big ? replay_candidates_g(A, K, nlists, s_subn, T0, reinterpret_cast<int*>(s_buf), s_i, s_sh, s_hist,
                                        cut, refined)
                  : replay_candidates(A, K, nlists, s_subn, T0, s_buf, s_i, s_sh, s_hist, cut, refined);
     };
-    rcode = replay(1, C->cand_t);
+    int nlists = 1;
+    double t_used = C->cand_t;
+    rcode = replay(nlists, t_used);
     const bool miss = rcode != RC_DONE;
     if (rcode == RC_FEW && C->cand_t2 < C->cand_t) {
       __syncthreads();
-      rcode = replay(2, C->cand_t2);
+      nlists = 2;
+      t_used = C->cand_t2;
+      rcode = replay(nlists, t_used);
+    }
+    if (rcode == RC_MANY && !big && A.rsg != nullptr) {
+      __syncthreads();
+      big = true;
+      rcode = replay(nlists, t_used);
     }
     if (rcode == RC_DONE) {
       if (tid == 0) C->ncopy = K;
