@@ -10,7 +10,8 @@ population entering each step, divided by the wall time between two device syncs
 
 Also reported: the same case with importance sampling (Gaussian-product psi_T, drift and
 local energy from the trial kernels, Metropolis); the continuous-weighting workload (100k
-walkers, 10 000 steps, exact split replay, descendant weighting) in FP32 and FP64; the
+walkers, 10 000 steps, exact split replay, descendant weighting, and the same at 1M walkers
+for 2 000 steps) in FP32 and FP64; the
 surrogate form (an MLP trained on the
 GPU on 2M geometries an analytic importance-sampled run visits, then driving the same
 importance-sampled DMC); ZPE runs of every form against FP64 runs; the dominant kernel's
@@ -671,6 +672,9 @@ def main():
         torch.cuda.empty_cache()
         log('[bench] continuous-weighting variant')
         variants['continuous'] = continuous_variant()
+        torch.cuda.empty_cache()
+        log('[bench] continuous-weighting variant at 1M walkers (16 000 kills per step)')
+        variants['continuous_1m'] = continuous_variant(1_000_000, 2000)
 
     if not args.no_variants and not args.no_surrogate:
         import pyvibdmc_amd as pv
