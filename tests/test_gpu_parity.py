@@ -203,6 +203,37 @@ def test_continuous_replay_from_a_million_walker_steady_state_matches_oracle():
     np.testing.assert_allclose(wts, orc.cont_wts, rtol=1e-12, atol=0)
 
 
+def test_continuous_replay_in_the_first_steps_of_a_million_walker_run_matches_oracle():
+    """1M walkers at the equilibrium geometry + N(0, 0.05 bohr), weights 1, FP64, the
+    reference's draws injected, 14 steps: the kills grow from none to a few thousand per step
+    while the weights are still bunched, so the candidate lists (threshold from the last
+    step's cut, or list B below it) hold tens of thousands of walkers.  The LDS form declines
+    them as too long and the global-memory form serves those steps (code 8) although the
+    kills fit the LDS sort; splits per step, vref, weights and coordinates match the oracle."""
+    import pyvibdmc_amd as pv
+    from oracle import pes as P
+    from oracle.dmc import LegacyDraws, OracleDMC
+    n, steps = 1_000_000, 14
+    masses = [1837.15, 1837.15, 29156.95]
+    start = P.water_eq()[None] + np.random.RandomState(0).normal(0.0, 0.05, (n, 3, 3))
+    kw = dict(masses=masses, start_structures=start, num_walkers=n, num_timesteps=steps, delta_t=5,
+              equil_steps=10 ** 9, wfn_every=10 ** 9, weighting='continuous', cont_wt_thresh=0.01)
+    orc = OracleDMC(potential=P.morse_bend, draws=LegacyDraws(21), **kw)
+    orc.propagate()
+    sim = pv.DMC_Sim(atoms=['H', 'H', 'O'], potential=pv.MorseBendTriatomic(), precision='fp64',
+                     rng=LegacyDraws(21), write_files=False, chkpt_every=10 ** 9, **kw)
+    sim.propagate()
+    nb, how = sim.trajectory_counts(0, steps)
+    print('splits', nb, 'replay', how, flush=True)
+    np.testing.assert_array_equal(nb, orc.rec['n_branched'])
+    small = (nb > 0) & (nb + nb // 4 <= 4096)
+    assert (small & (how == 8)).any(), (nb, how)
+    np.testing.assert_allclose(sim._vref_vs_tau, orc._vref_vs_tau, rtol=0, atol=1e-10)
+    coords, wts = sim._get_walkers()
+    np.testing.assert_array_equal(coords, orc.coords)
+    np.testing.assert_allclose(wts, orc.cont_wts, rtol=1e-12, atol=0)
+
+
 def test_continuous_replay_with_a_cut_inside_the_list_matches_oracle():
     """A tenth of the walkers start in one dense cluster of near-equal heavy weights (5.0,
     spread 1e-6) and 3 % below the threshold.  After the first step (interval replay) the
