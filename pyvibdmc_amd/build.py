@@ -73,5 +73,37 @@ def build(verbose=True, only=None):
     return outs
 
 
+# Host AddressSanitizer build of the C ABI layer: pvd_capi.hip's host code instrumented (the
+# entry points, argument checks, allocation and clean-up paths), linked with the step and
+# surrogate objects of build() and the shared ASan runtime.  Device code is not instrumented.
+ASAN_LIB = os.path.join(OBJ, 'libpyvibdmc_amd_asan.so')
+ASAN_FLAGS = ['-Xarch_host', '-fsanitize=address', '-Xarch_host', '-fno-omit-frame-pointer']
+
+
+def asan_runtime():
+    """The shared ASan runtime of ROCm's clang (for LD_PRELOAD), or None."""
+    import glob
+    hits = sorted(glob.glob('/opt/rocm/llvm/lib/clang/*/lib/linux/libclang_rt.asan-x86_64.so'))
+    return hits[-1] if hits else None
+
+
+def build_asan(verbose=True):
+    """Build ASAN_LIB (after build(), whose other objects it links); returns its path."""
+    others = [os.path.join(OBJ, s) for s in ('pvd_step_f32.o', 'pvd_step_f64.o', 'pvd_mlp.o')]
+    if not all(os.path.exists(o) for o in others):
+        build(verbose)
+    obj = os.path.join(OBJ, 'pvd_capi_asan.o')
+    _run([hipcc()] + FLAGS + ASAN_FLAGS + ['-c', os.path.join(CSRC, 'pvd_capi.hip'), '-o', obj], 'pvd_capi.hip (asan)',
+         verbose)
+    tmp = ASAN_LIB + '.tmp'
+    _run([hipcc(), f'--offload-arch={ARCH}', '-shared', '-fPIC'] + ASAN_FLAGS + ['-shared-libasan', '-o', tmp, obj] +
+         others, 'libpyvibdmc_amd_asan.so', verbose)
+    os.replace(tmp, ASAN_LIB)
+    return ASAN_LIB
+
+
 if __name__ == '__main__':
-    build(only=sys.argv[1:] or None)
+    if sys.argv[1:] == ['asan']:
+        build_asan()
+    else:
+        build(only=sys.argv[1:] or None)
