@@ -93,17 +93,17 @@ struct Cand {
   int pad;
 };
 constexpr int CAND_CAP = 4096;   // candidates the replay sorts in LDS
-constexpr int CAND_G = 32768;    // candidates the global-memory replay sorts (more kills per step)
+constexpr int CAND_G = 131072;   // candidates the global-memory replay sorts (2 KILL_CAP)
 // scratch of the global-memory replay: keys and bucketed keys (8 B each), order and bucketed
 // order (4 B each), family starts (4 B), split depths (1 B) per candidate
 constexpr size_t RSG_BYTES = (size_t)CAND_G * (8 + 8 + 4 + 4 + 4 + 1);
 constexpr int LIST_SUB = 8;       // sub-lists per list (spreads the appends' counters)
-constexpr int SUB_CAP = 16384;    // entries per sub-list and step
+constexpr int SUB_CAP = 32768;    // entries per sub-list and step
 constexpr int LIST_CAP = LIST_SUB * SUB_CAP;
 // sub-list entries the LDS replay reads: it keeps a candidate's position in 16 bits, the
 // sub-list (4 bits) and the offset in it (12 bits)
 constexpr int SUB_CAP16 = 4096;
-constexpr int KILL_CAP = 32768;  // kills per step
+constexpr int KILL_CAP = 65536;  // kills per step
 
 struct Piece {
   double v;
@@ -1984,8 +1984,10 @@ PVD_DEV int replay_candidates_g(const StepArgs<R>& A, int K, int nlists, const i
       for (int u = 0; u < U; ++u) kk[u] = jb + u < j1 ? key[jb + u] : 0ull;
 #pragma unroll
       for (int u = 0; u < U; ++u)
-        if (jb + u < j1) loc += fam(kk[u], E);
+        if (jb + u < j1) loc = min(loc + fam(kk[u], E), KILL_CAP + 1);
     }
+    // a thread's count is clamped above K: the block sums stay in int, and every later
+    // thread still starts at or beyond K
     int tot;
     const int ex = block_excl_scan<NT>(loc, s_i, tot);
     int st = base + ex;
@@ -2605,7 +2607,7 @@ __global__ __launch_bounds__(RP_NT) void k_cont_replay(StepArgs<R> A) {
 // energy, descendant index).  Sources are never kill slots, so the copies are independent
 // and spread over the whole chip.  After the global-memory replay (gc_n > 0) the copy of
 // kill index t takes its family from asg[t] and its slot from klist[t], and writes the kill
-// slot's weight; thread t also writes sorted candidate t's own weight when it was split.
+// slot's weight; the threads also write the split candidates' own weights (grid-strided).
 // Kill slots, split candidates and family p's slots are distinct walkers, so every weight
 // written here has one writer.
 template <typename R>
@@ -2619,9 +2621,10 @@ __global__ __launch_bounds__(TILE) void k_cont_copy(StepArgs<R> A) {
   if (gn > 0) {
     const RsgView rv = rsg_view(A.rsg);
     const int gp = C->gc_p;
-    if (t < gn && t != gp) {
-      const int d = rv.dep[t];
-      if (d > 0) A.w[rv.ord[t]] = ldexp(key_weight(rv.key[t]), -d);
+    for (int j = t; j < gn; j += (int)(gridDim.x * TILE)) {  // the candidates may outnumber the threads
+      if (j == gp) continue;
+      const int d = rv.dep[j];
+      if (d > 0) A.w[rv.ord[j]] = ldexp(key_weight(rv.key[j]), -d);
     }
     if (t >= C->ncopy) return;
     ks = A.klist[t];

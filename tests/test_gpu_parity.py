@@ -203,6 +203,44 @@ def test_continuous_replay_from_a_million_walker_steady_state_matches_oracle():
     np.testing.assert_allclose(wts, orc.cont_wts, rtol=1e-12, atol=0)
 
 
+@pytest.mark.timeout(600)
+def test_continuous_replay_beyond_32768_kills_per_step_matches_oracle():
+    """2M walkers, FP64, the reference's draws injected, the spread start weights of the
+    600k case (log-uniform over [e^-2, e^2], 20 % in a band just above the threshold): about
+    48 000 kills in the first step (interval replay) and 38 000 in the second (global-memory
+    candidate replay), both above the 32 768 a step was limited to before; splits per step,
+    vref, weights and coordinates match the oracle's sequential loop."""
+    import pyvibdmc_amd as pv
+    from oracle import pes as P
+    from oracle.dmc import LegacyDraws, OracleDMC
+    n, steps = 2_000_000, 2
+    masses = [1837.15, 1837.15, 29156.95]
+    rs = np.random.RandomState(23)
+    start = P.water_eq()[None] + rs.normal(0, 0.05, (n, 3, 3))
+    w0 = np.exp(rs.uniform(-2.0, 2.0, n))
+    band = rs.rand(n) < 0.2
+    w0[band] = 0.01 * np.exp(rs.uniform(0.0, 0.1, int(band.sum())))
+    w0[~band] *= (n - w0[band].sum()) / w0[~band].sum()
+    kw = dict(masses=masses, start_structures=start, num_walkers=n, num_timesteps=steps, delta_t=5,
+              equil_steps=10 ** 9, wfn_every=10 ** 9, weighting='continuous', cont_wt_thresh=0.01,
+              start_cont_wts=w0)
+    sim = pv.DMC_Sim(atoms=['H', 'H', 'O'], potential=pv.MorseBendTriatomic(), precision='fp64',
+                     rng=LegacyDraws(9), write_files=False, chkpt_every=10 ** 9, **kw)
+    sim.propagate()
+    nb, how = sim.trajectory_counts(0, steps)
+    print('splits', nb, 'replay', how, flush=True)
+    assert nb.min() > 32768 and how[0] == 2 and how[1] in (7, 8), (nb, how)
+    coords, wts = sim._get_walkers()
+    vref = sim._vref_vs_tau.copy()
+    del sim
+    orc = OracleDMC(potential=P.morse_bend, draws=LegacyDraws(9), **kw)
+    orc.propagate()
+    np.testing.assert_array_equal(nb, orc.rec['n_branched'])
+    np.testing.assert_allclose(vref, orc._vref_vs_tau, rtol=0, atol=1e-10)
+    np.testing.assert_array_equal(coords, orc.coords)
+    np.testing.assert_allclose(wts, orc.cont_wts, rtol=1e-12, atol=0)
+
+
 def test_continuous_replay_in_the_first_steps_of_a_million_walker_run_matches_oracle():
     """1M walkers at the equilibrium geometry + N(0, 0.05 bohr), weights 1, FP64, the
     reference's draws injected, 14 steps: the kills grow from none to a few thousand per step
