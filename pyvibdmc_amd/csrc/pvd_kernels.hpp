@@ -686,7 +686,11 @@ __global__ __launch_bounds__(TILE) void k_discrete(StepArgs<R> A) {
   double dte = A.dt;
   if (IS) dte = A.dt * ((double)acc_total(A, par) / (double)n);
 
-  R x[W][EXT ? 1 : NP];
+  // external energies: walkers of at most 36 bytes are loaded with their energies, before the
+  // look-back, and written from registers (loading them after it put the read latency
+  // between the prefix and the writes); larger ones are copied plane by plane in the write loop
+  constexpr bool PREF = EXT && NP > 0 && (int)sizeof(R) * NP <= 36;
+  R x[W][(!EXT || PREF) ? (NP > 0 ? NP : 1) : 1];
   int c[W];
   double sce = 0.0;
   int births = 0, deaths = 0, bad = 0;
@@ -700,6 +704,7 @@ __global__ __launch_bounds__(TILE) void k_discrete(StepArgs<R> A) {
       if constexpr (EXT) {
         E = A.V_ext[i];
         if (IS) E = E + A.ke[i];
+        if constexpr (PREF) load_walker<R, NP>(A, i, x[k]);
       } else if constexpr (IS) {
         // moved, and its local energy computed, by k_is_move
         load_walker<R, NP>(A, i, x[k]);
@@ -826,7 +831,7 @@ __global__ __launch_bounds__(TILE) void k_discrete(StepArgs<R> A) {
         if (dst >= A.cap) break;
         if constexpr (EXT && NP == 0) {
           for (int p = 0; p < A.np; ++p) A.cout[p * A.cap + dst] = A.cin[p * A.cap + i];
-        } else if constexpr (EXT) {
+        } else if constexpr (EXT && !PREF) {
           // external energies: the walker is copied from the input planes
           if constexpr (sizeof(R) == 4) {
             const __amdgpu_buffer_rsrc_t ri = coord_rsrc((const float*)A.cin, A.cap, NP);
