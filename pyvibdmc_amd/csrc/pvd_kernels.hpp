@@ -1689,10 +1689,13 @@ static_assert(RCG_BUCKETS * (int)sizeof(int) <= RP_LDS && RCG_BUCKETS % RP_NT ==
 
 // The same closed-form replay when the kills outgrow the LDS sort (1.25 K > CAND_CAP): up to
 // CAND_G candidates, sorted and scanned through an L2-resident scratch (A.rsg) instead of LDS
-// and registers.  One workgroup runs it, so each pass over the scratch is bound by the latency
-// of its loads: every streaming loop issues RG_U independent loads per thread before it uses
-// them, and the octave scans give each thread a contiguous run of candidates (one block scan
-// per octave).  The sort key holds the weight's exponent and mantissa whole (the weight is
+// and registers.  One workgroup runs it, so each pass over the scratch is bound by what one CU
+// issues: the latency of its loads, and for scattered addresses about one cache line per
+// cycle.  Every streaming loop issues RG_U independent loads per thread before it uses them,
+// the octave scans give each thread a contiguous run of candidates (one block scan per
+// octave), and the scattered weight and copy writes are the chip-wide copy kernel's.  Equal
+// keys are ranked by position; where that order would matter (a tie in an octave that is
+// used) the replay declines.  The sort key holds the weight's exponent and mantissa whole (the weight is
 // rebuilt from it) and the candidate's walker slot travels with it, so no pass after the
 // collection reads the lists again.  Bucket counts of the counting sort live in LDS (`bcnt`,
 // RCG_BUCKETS ints).  It declines where the LDS form declines, and also when one mantissa
