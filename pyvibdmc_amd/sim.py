@@ -339,8 +339,9 @@ class DMC_Sim:
 
     def trajectory_counts(self, t0, t1):
         """births/deaths per step (discrete), or number of splits and the split replay that ran
-        (continuous: 0 none; candidate lists: 1 list A, 5 list A cut, 6 lists A and B;
-        2-4 interval replay: lists too short / too long / other)."""
+        (continuous: 0 none; LDS candidate replay: 1 list A, 5 list A cut, 6 lists A and B;
+        global-memory candidate replay: 7 list A, 8 after list A alone fell short or the LDS
+        form declined its lists; 2-4 interval replay: lists too short / too long / other)."""
         cnt = t1 - t0
         b = np.empty(cnt, dtype=np.int32)
         dd = np.empty(cnt, dtype=np.int32)
