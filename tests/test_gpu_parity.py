@@ -203,6 +203,47 @@ def test_continuous_replay_from_a_million_walker_steady_state_matches_oracle():
     np.testing.assert_allclose(wts, orc.cont_wts, rtol=1e-12, atol=0)
 
 
+@pytest.mark.timeout(300)
+def test_continuous_importance_sampled_replay_at_scale_matches_oracle():
+    """600k walkers, FP64, importance sampling with the water Gaussian-product trial, the
+    reference's draws injected; half the start weights in a band a hundredth of an e-fold
+    above the threshold, so that 20 000 - 34 000 walkers cross it per step (the local energy
+    keeps the weight factors near 1): the interval replay in the first step, the
+    global-memory candidate replay after.  Splits per step, vref and effective time steps
+    match the oracle; coordinates to 1e-9 bohr and weights to 1e-10 (the analytic drift
+    differs from the reference's in the last ulps, as in the golden IS cases)."""
+    import pyvibdmc_amd as pv
+    from oracle import pes as P
+    from oracle.dmc import LegacyDraws, OracleDMC
+    n, steps = 600_000, 3
+    masses = [1837.15, 1837.15, 29156.95]
+    rs = np.random.RandomState(29)
+    start = P.water_eq()[None] + rs.normal(0, 0.05, (n, 3, 3))
+    w0 = np.exp(rs.uniform(-2.0, 2.0, n))
+    band = rs.rand(n) < 0.5
+    w0[band] = 0.01 * np.exp(rs.uniform(0.0, 0.01, int(band.sum())))
+    w0[~band] *= (n - w0[band].sum()) / w0[~band].sum()
+    kw = dict(masses=masses, start_structures=start, num_walkers=n, num_timesteps=steps, delta_t=5,
+              equil_steps=10 ** 9, wfn_every=10 ** 9, weighting='continuous', cont_wt_thresh=0.01,
+              start_cont_wts=w0)
+    sim = pv.DMC_Sim(atoms=['H', 'H', 'O'], potential=pv.MorseBendTriatomic(), imp_samp=pv.water_trial(),
+                     precision='fp64', rng=LegacyDraws(31), write_files=False, chkpt_every=10 ** 9, **kw)
+    sim.propagate()
+    nb, how = sim.trajectory_counts(0, steps)
+    print('splits', nb, 'replay', how, flush=True)
+    assert how[0] == 2 and np.isin(how[1:], [7, 8]).all(), (nb, how)
+    coords, wts = sim._get_walkers()
+    vref, eff = sim._vref_vs_tau.copy(), np.asarray(sim.eff_ts).copy()
+    del sim
+    orc = OracleDMC(potential=P.morse_bend, draws=LegacyDraws(31), trial=P.water_trial(), **kw)
+    orc.propagate()
+    np.testing.assert_array_equal(nb, orc.rec['n_branched'])
+    np.testing.assert_allclose(vref, orc._vref_vs_tau, rtol=0, atol=1e-10)
+    np.testing.assert_allclose(eff, orc.eff_ts, rtol=0, atol=1e-12)
+    np.testing.assert_allclose(coords, orc.coords, rtol=0, atol=1e-9)
+    np.testing.assert_allclose(wts, orc.cont_wts, rtol=1e-10, atol=0)
+
+
 @pytest.mark.timeout(600)
 def test_continuous_replay_beyond_32768_kills_per_step_matches_oracle():
     """2M walkers, FP64, the reference's draws injected, the spread start weights of the
