@@ -1936,9 +1936,15 @@ PVD_DEV int replay_candidates_g(const StepArgs<R>& A, int K, int nlists, const i
 #pragma unroll
     for (int u = 0; u < U; ++u) {
       const int q = q0 + u * NT + tid;
-      for (int x = bs[u] + RK; x < be[u]; ++x) {
-        const unsigned long long ki = bkey[x];
-        rank[u] += (ki < kq[u] || (ki == kq[u] && x < q)) ? 1 : 0;
+      for (int x0 = bs[u] + RK; x0 < be[u]; x0 += 8) {  // larger buckets: eight loads in flight
+        unsigned long long k8[8];
+#pragma unroll
+        for (int v = 0; v < 8; ++v) k8[v] = x0 + v < be[u] ? bkey[x0 + v] : ~0ull;
+#pragma unroll
+        for (int v = 0; v < 8; ++v) {
+          const int x = x0 + v;
+          rank[u] += (x < be[u] && (k8[v] < kq[u] || (k8[v] == kq[u] && x < q))) ? 1 : 0;
+        }
       }
       if (q < C) {
         key[bs[u] + rank[u]] = kq[u];
