@@ -240,6 +240,38 @@ def test_c1_continuous_fp32_with_descendant_weighting_tracks_fp64():
     assert abs(r32 - r64) < 2e-3 and 1.80 < r64 < 1.90, out
 
 
+@pytest.mark.timeout(300)
+def test_continuous_fp32_at_a_million_walkers_tracks_fp64():
+    """Continuous weighting at 1M walkers (about 16 000 kills per step, the global-memory
+    candidate replay on nearly every step), dt = 5, 8 000 steps: the FP32 propagation and the
+    FP64 run give the same ZPE within 3 combined blocking errors, and both sit on the plain
+    dt = 5 value of this surface (4676.5 cm^-1, FP64 discrete at 1M walkers) within 2 cm^-1."""
+    import pyvibdmc_amd as pv
+    WN = 4.556335281212229e-6
+    n, T = 1_000_000, 8000
+    start = P.water_eq()[None] + np.random.RandomState(9).normal(0.0, 0.05, (n, 3, 3))
+    out = {}
+    for prec in ('fp32', 'fp64'):
+        sim = pv.DMC_Sim(num_walkers=n, num_timesteps=T, atoms=['H', 'H', 'O'], masses=[P.M_H, P.M_H, P.M_O],
+                         delta_t=5, potential=pv.MorseBendTriatomic(), start_structures=start,
+                         weighting='continuous', cont_wt_thresh=0.01, equil_steps=10 ** 9, wfn_every=10 ** 9,
+                         chkpt_every=10 ** 9, precision=prec, seed=9, write_files=False)
+        sim.propagate()
+        nb, how = sim.trajectory_counts(0, T)
+        assert np.isin(how[T // 2:], [7, 8]).mean() > 0.99, np.unique(how, return_counts=True)
+        v = sim._vref_vs_tau[T // 4:] / WN
+        se, b = 0.0, 1
+        while len(v) // b >= 16:
+            m = len(v) // b
+            se = max(se, v[:m * b].reshape(m, b).mean(axis=1).std(ddof=1) / np.sqrt(m))
+            b *= 2
+        out[prec] = (v.mean(), se)
+        del sim
+    (z32, s32), (z64, s64) = out['fp32'], out['fp64']
+    assert abs(z32 - z64) < 3 * np.hypot(s32, s64), out
+    assert abs(z64 - 4676.5) < 2.0 and abs(z32 - 4676.5) < 2.0, out
+
+
 def test_is_fp32_cluster_matches_fp64_zpe():
     """Six-atom cluster with the per-monomer trial: the FP32 move (gradients + closed-form
     weighted Laplacian, runtime bond/angle lists) and the FP64 move (per-component
