@@ -236,7 +236,7 @@ def continuous_variant(n=100_000, T=10_000):
         out[prec] = {"value": n * T / wall, "unit": "walker-steps/s", "ms_per_step": wall * 1e3 / T,
                      "zpe_cm": z, "zpe_se_cm": se, "splits_per_step": float(np.mean(nb)),
                      "replay_paths": {str(int(k)): int(v) for k, v in zip(*np.unique(how, return_counts=True))},
-                     "wfns": len(sim.wfns), "step_kernels_us": kus,
+                     "wfns": len(getattr(sim, 'wfns', [])), "step_kernels_us": kus,
                      "k_cont_roofline": {"bound": "hbm", "achieved": ach, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                                          "frac": ach / HBM_PEAK_GBS, "bytes_per_walker_step": bpw}}
         log(f'[bench] continuous {prec}: {n * T / wall:.4g} walker-steps/s, {wall * 1e6 / T:.1f} us/step, '
