@@ -272,6 +272,29 @@ def test_continuous_fp32_at_a_million_walkers_tracks_fp64():
     assert abs(z64 - 4676.5) < 2.0 and abs(z32 - 4676.5) < 2.0, out
 
 
+def test_continuous_step_beyond_the_kill_limit_stops_the_run():
+    """1.5M walkers, half of them with weights in a band just above the 0.01 threshold: about
+    90 000 walkers cross it in the first step, more than the 65 536 a step may split.  The
+    replay sets the status without touching the walkers, every later kernel of the step returns
+    on it, and the run stops with a DMCError naming the split, at step 0."""
+    import pyvibdmc_amd as pv
+    from pyvibdmc_amd._native import DMCError
+    n = 1_500_000
+    rs = np.random.RandomState(41)
+    start = P.water_eq()[None] + rs.normal(0.0, 0.05, (n, 3, 3))
+    w0 = np.exp(rs.uniform(-2.0, 2.0, n))
+    band = rs.rand(n) < 0.5
+    w0[band] = 0.01 * np.exp(rs.uniform(0.0, 0.1, int(band.sum())))
+    w0[~band] *= (n - w0[band].sum()) / w0[~band].sum()
+    sim = pv.DMC_Sim(num_walkers=n, num_timesteps=5, atoms=['H', 'H', 'O'], masses=[P.M_H, P.M_H, P.M_O],
+                     delta_t=5, potential=pv.MorseBendTriatomic(), start_structures=start, start_cont_wts=w0,
+                     weighting='continuous', cont_wt_thresh=0.01, equil_steps=10 ** 9, wfn_every=10 ** 9,
+                     chkpt_every=10 ** 9, precision='fp32', seed=3, write_files=False)
+    with pytest.raises(DMCError, match='continuous split outside'):
+        sim.propagate()
+    assert sim.cur_timestep == 0
+
+
 def test_is_fp32_cluster_matches_fp64_zpe():
     """Six-atom cluster with the per-monomer trial: the FP32 move (gradients + closed-form
     weighted Laplacian, runtime bond/angle lists) and the FP64 move (per-component
