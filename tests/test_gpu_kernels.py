@@ -184,6 +184,38 @@ def test_graph_replay_matches_eager_launches(monkeypatch, mode):
         np.testing.assert_array_equal(w1, w2)
 
 
+@pytest.mark.timeout(300)
+def test_plain_fp32_production_step_tracks_fp64_at_a_million_walkers():
+    """The headline step as it runs in production: water (triatomic Morse+bend), 1M walkers,
+    dt = 5, discrete branching, Philox draws, the compact FP32 Box-Muller layout and the FP32
+    branching weight (k_discrete<float,9>), 6 000 steps, against the FP64 run (FP64 draws and
+    weights).  Same ZPE within 3 combined blocking errors, both within 2 cm^-1 of this surface's
+    dt = 5 value (4676.5 cm^-1), and the population within 2 % of N0 throughout."""
+    import pyvibdmc_amd as pv
+    WN = 4.556335281212229e-6
+    n, T = 1_000_000, 6000
+    start = P.water_eq()[None] + np.random.RandomState(12).normal(0.0, 0.05, (n, 3, 3))
+    out = {}
+    for prec in ('fp32', 'fp64'):
+        sim = pv.DMC_Sim(num_walkers=n, num_timesteps=T, atoms=['H', 'H', 'O'], masses=[P.M_H, P.M_H, P.M_O],
+                         delta_t=5, potential=pv.MorseBendTriatomic(), start_structures=start, equil_steps=10 ** 9,
+                         wfn_every=10 ** 9, chkpt_every=10 ** 9, precision=prec, seed=12, write_files=False)
+        sim.propagate()
+        pop = np.asarray(sim._pop_vs_tau[:T], float)
+        assert np.abs(pop / n - 1.0).max() < 0.02, (prec, pop.min(), pop.max())
+        v = sim._vref_vs_tau[T // 4:] / WN
+        se, b = 0.0, 1
+        while len(v) // b >= 16:
+            m = len(v) // b
+            se = max(se, v[:m * b].reshape(m, b).mean(axis=1).std(ddof=1) / np.sqrt(m))
+            b *= 2
+        out[prec] = (v.mean(), se)
+        del sim
+    (z32, s32), (z64, s64) = out['fp32'], out['fp64']
+    assert abs(z32 - z64) < 3 * np.hypot(s32, s64), out
+    assert abs(z64 - 4676.5) < 2.0 and abs(z32 - 4676.5) < 2.0, out
+
+
 def test_is_fp32_fast_trial_path_matches_fp64_zpe():
     """The FP32 importance-sampled move on the [H, H, O] trial uses gradients and a
     closed-form mass-weighted Laplacian; FP64 uses the per-component derivatives of the
