@@ -742,7 +742,8 @@ def main():
            "ms_per_step": ms_per_step, "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
            "dtype": "f32" if args.precision == 'fp32' else "f64", "data": "synthetic",
            "config": {"workload": f"plain DMC, triatomic Morse+bend PES, {n} walkers, dt=5, discrete "
-                                  f"birth/death, {args.precision} coordinates with FP64 weights/reductions, "
+                                  f"birth/death, {args.precision} coordinates and branching weights "
+                                  f"(exponent argument in FP64), FP64 reductions (vref, population), "
                                   f"Philox draws",
                       "walkers": n, "mean_population": mean_n, "parallelism": "single GPU"},
            "roofline": roof, "cpu_baseline": cpu, "variants": variants, "zpe": zpe, "time_to_accuracy": tta}
