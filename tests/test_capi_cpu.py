@@ -50,3 +50,50 @@ def test_bad_arguments_are_rejected_without_gpu_work():
     cfg.num_walkers = 0
     h = N.ctypes_void()
     assert lib.pvd_sim_create(N.byref(cfg), N.byref(h)) == N.E_BAD
+
+
+C_PROBE = r'''
+#include <stddef.h>
+#include <stdio.h>
+#include "pyvibdmc_amd.h"
+int main(void) {
+  printf("%zu %zu %zu %zu\n", sizeof(pvd_config), sizeof(pvd_pes_desc), sizeof(pvd_trial_desc), sizeof(pvd_mlp));
+  printf("%zu %zu %zu %zu %zu\n", offsetof(pvd_config, delta_t), offsetof(pvd_config, sigma),
+         offsetof(pvd_config, seed), offsetof(pvd_config, pes), offsetof(pvd_config, trial));
+  printf("%zu %zu %zu\n", offsetof(pvd_mlp, xmean), offsetof(pvd_mlp, quad), offsetof(pvd_mlp, params));
+  printf("%s\n", pvd_version());
+  printf("%d\n", pvd_sim_create(NULL, NULL));
+  printf("%s\n", pvd_last_error());
+  return 0;
+}
+'''
+
+
+def test_header_compiles_as_c_and_matches_the_ctypes_layouts(tmp_path):
+    """A C11 program built with gcc against include/pyvibdmc_amd.h and linked with the library:
+    the header is plain C, the structure sizes and field offsets the C compiler lays out are
+    the ones the Python bindings declare, and the entry points that need no GPU answer."""
+    import ctypes
+    import shutil
+    import subprocess
+    from pyvibdmc_amd import _native as N
+    from pyvibdmc_amd import surrogate as S
+    if not os.path.exists(N.lib_path()) or shutil.which('gcc') is None:
+        pytest.skip('library not built or no C compiler')
+    src, exe = tmp_path / 'probe.c', tmp_path / 'probe'
+    src.write_text(C_PROBE)
+    libdir = os.path.dirname(N.lib_path())
+    subprocess.run(['gcc', '-std=c11', '-Wall', '-Werror', '-I', os.path.join(ROOT, 'include'), str(src), '-o', str(exe),
+                    '-L', libdir, '-lpyvibdmc_amd', f'-Wl,-rpath,{libdir}'], check=True)
+    out = subprocess.run([str(exe)], capture_output=True, text=True, check=True,
+                         env=dict(os.environ, HIP_VISIBLE_DEVICES='')).stdout.splitlines()
+    sizes = [int(v) for v in out[0].split()]
+    assert sizes == [ctypes.sizeof(N.Config), ctypes.sizeof(N.PesDesc), ctypes.sizeof(N.TrialDesc),
+                     ctypes.sizeof(S.MLPDesc)], sizes
+    offs = [int(v) for v in out[1].split()]
+    assert offs == [N.Config.delta_t.offset, N.Config.sigma.offset, N.Config.seed.offset, N.Config.pes.offset,
+                    N.Config.trial.offset], offs
+    moffs = [int(v) for v in out[2].split()]
+    assert moffs == [S.MLPDesc.xmean.offset, S.MLPDesc.quad.offset, S.MLPDesc.params.offset], moffs
+    assert out[3].startswith('pyvibdmc_amd')
+    assert int(out[4]) == N.E_BAD and 'null' in out[5]
