@@ -346,3 +346,30 @@ def test_surrogate_on_the_partridge_schwenke_surface():
             assert max(rec[1] for rec in pot.judge_log) / WN < 10.0
     assert 4600 < z['analytic'] < 4680, z
     assert abs(z['surrogate'] - z['analytic']) < 1.0, z
+
+
+@pytest.mark.parametrize('hidden', [128, 256])
+def test_walkers_outside_the_training_range_get_the_analytic_potential(hidden):
+    """The inference kernels (H = 128 and the staged H = 256 one) with the range check on: a
+    walker whose descriptor leaves the normalisation range in any component gets the fallback
+    surface's value (the analytic triatomic PES, FP64 planes: the oracle's value to 1e-12), and
+    every other walker gets exactly the network's value with the check off."""
+    import pyvibdmc_amd as pv
+    rs = np.random.RandomState(8)
+    m = pv.SurrogateModel(WATER, hidden=hidden, seed=2)
+    near = P.water_eq()[None] + rs.normal(0.0, 0.02, (20000, 3, 3))
+    m.set_normalisation(near, P.morse_bend(near))
+    x = np.concatenate([P.water_eq()[None] + rs.normal(0.0, 0.01, (3000, 3, 3)),
+                        P.water_eq()[None] + rs.normal(0.0, 0.3, (3000, 3, 3))])
+    f = m.features(x, normalise=False).cpu().numpy()  # (D, n)
+    d = m.desc
+    lo = np.array([d.lo[k] for k in range(m.D)], np.float32)[:, None]
+    hi = np.array([d.hi[k] for k in range(m.D)], np.float32)[:, None]
+    out = ((f < lo) | (f > hi)).any(axis=0)
+    assert 0.1 * len(x) < out.sum() < 0.9 * len(x), out.sum()
+    d.use_range = 1
+    v_fb = m.energy(x, pv.MorseBendTriatomic())
+    d.use_range = 0
+    v_net = m.energy(x)
+    np.testing.assert_allclose(v_fb[out], P.morse_bend(x[out]), rtol=1e-12, atol=1e-15)
+    np.testing.assert_array_equal(v_fb[~out], v_net[~out])
