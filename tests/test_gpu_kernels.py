@@ -156,6 +156,35 @@ def test_restart_from_checkpoint_continues_the_same_draws(tmp_path):
     np.testing.assert_allclose(b._vref_vs_tau, c._vref_vs_tau, rtol=0, atol=1e-12)
 
 
+def test_continuous_restart_from_checkpoint_continues_the_same_run(tmp_path):
+    """Continuous weighting (20 000 walkers, FP64, threshold 0.01): a restart from the step-100
+    checkpoint reproduces the uninterrupted run: same walkers, weights and vref.  The restarted
+    run's first step has no candidate list yet and takes the interval replay, which is exact
+    as well; the sums around the split may round differently there, hence vref to 1e-12 and
+    the weights to 1e-12 relative."""
+    import pyvibdmc_amd as pv
+    n = 20000
+    kw = dict(num_walkers=n, atoms=['H', 'H', 'O'], masses=[1837.15, 1837.15, 29156.95], delta_t=5,
+              start_structures=water(n, 8, 0.05), precision='fp64', seed=43, equil_steps=10 ** 9,
+              wfn_every=10 ** 9, weighting='continuous', cont_wt_thresh=0.01, output_folder=str(tmp_path))
+    a = pv.DMC_Sim(sim_name='a', num_timesteps=200, chkpt_every=100, potential=pv.MorseBendTriatomic(), **kw)
+    a.propagate()
+    b = pv.dmc_restart(pv.MorseBendTriatomic(), str(tmp_path), 'a', additional_timesteps=100)
+    assert b.cur_timestep == 100
+    b.propagate()
+    c = pv.DMC_Sim(sim_name='c', num_timesteps=300, chkpt_every=10 ** 9, potential=pv.MorseBendTriatomic(), **kw)
+    c.propagate()
+    nb_b, _ = b.trajectory_counts(100, 300)
+    nb_c, _ = c.trajectory_counts(100, 300)
+    assert nb_c.sum() > 1000
+    np.testing.assert_array_equal(nb_b, nb_c)
+    np.testing.assert_allclose(b._vref_vs_tau, c._vref_vs_tau, rtol=0, atol=1e-12)
+    cb, wb = b._get_walkers()
+    cc, wc = c._get_walkers()
+    np.testing.assert_array_equal(cb, cc)
+    np.testing.assert_allclose(wb, wc, rtol=1e-12, atol=0)
+
+
 @pytest.mark.parametrize('mode', ['discrete', 'continuous', 'is'])
 def test_graph_replay_matches_eager_launches(monkeypatch, mode):
     """pvd_sim_run replays a 64-step HIP graph; 300 steps (4 replays + remainder) must give
